@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the masked 3D diffusion on one MI355X.
+
+A step is one full diffusion of the workload (all repetitions) with the mask
+and both state buffers already resident in HBM.  Default workload: 1024^3
+mask (cylinder r=300 + 2000 spheres r~U[5,40]), sigma = 10 (R = 40, 81 taps),
+40 repetitions, FP32 state; the result is written as uint16.
+
+Prints ONE JSON line.  `value` is Gvoxel*iterations/s over the K timed steps.
+`roofline` describes the dominant kernel: its average launch duration is
+measured with HIP events recorded on the launch stream inside the timed
+region.  `cpu_baseline` times the reference's own CPU module (built from its
+sources into oracle/_ref) on a bounded crop of the same workload.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+PEAK_MFMA = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}  # dense TFLOP/s
+PEAK_VALU_FP32 = 78.6  # non-packed FP32 FMA, TFLOP/s
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--workload", default="c2", choices=["c1", "c2", "c4"])
+    p.add_argument("--precision", default="fp32",
+                   choices=["fp32", "bf16", "fp16", "fp32_fma", "fp32_exact"])
+    p.add_argument("--reps", type=int, default=None, help="override the workload's repetitions")
+    p.add_argument("--cpu-seconds", type=float, default=15.0,
+                   help="target CPU-baseline duration (0 disables it)")
+    return p.parse_args()
+
+
+def algorithmic(workload, precision):
+    """Bytes and useful flops per voxel for one pass of each kernel class."""
+    sb = 2 if precision in ("bf16", "fp16") else 4
+    K = 2 * workload.radius + 1
+    return {"zpass": (2 * sb, 2 * K), "ypass": (2 * sb, 2 * K), "xpass": (2 * sb + 1, 2 * K)}
+
+
+def cpu_baseline(workload, target_s):
+    """Time the reference CPU module on a central crop of the workload mask."""
+    import oracle
+    ref = oracle.ref_module()
+    kind = "reference"
+    if ref is None:
+        kind = "port"
+    mask = workload.mask()
+    c = [min(s, 256) for s in mask.shape]
+    o = [(s - cc) // 2 for s, cc in zip(mask.shape, c)]
+    crop = np.ascontiguousarray(mask[o[0]:o[0] + c[0], o[1]:o[1] + c[1], o[2]:o[2] + c[2]])
+    k = workload.kernel().astype(np.float32)
+    out = np.empty(crop.shape, np.uint16)
+
+    def run(reps):
+        t = time.perf_counter()
+        if ref is not None:
+            ref.diffusion(crop, k, out, reps)
+        else:
+            oracle.diffusion_f32(crop, k, reps)
+        return time.perf_counter() - t
+
+    t1 = run(1)
+    reps = int(max(1, min(workload.reps, round(target_s / max(t1, 1e-3)))))
+    t = run(reps)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": crop.size * reps / t / 1e9, "unit": "Gvoxel*iter/s", "cores": cores,
+            "kind": kind,
+            "sample": f"central {c[0]}x{c[1]}x{c[2]} crop of the workload mask, "
+                      f"{len(k)} taps, {reps} repetitions, {t:.1f} s"}
+
+
+def load_traffic(workload_name, precision, kernel):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))
+        return d.get(f"{workload_name}/{precision}/{kernel}")
+    except (ValueError, OSError):
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from maxibone_amd import capi
+    from maxibone_amd.synth import WORKLOADS
+
+    w = WORKLOADS[args.workload]
+    reps = w.reps if args.reps is None else args.reps
+    L = capi.lib()
+    dev = torch.device("cuda:0")
+    mask_h = w.mask()
+    nz, ny, nx = mask_h.shape
+    n = mask_h.size
+    k = w.kernel().astype(np.float32)
+    prec = capi.PRECISION[args.precision]
+    sbytes = L.mxb_state_bytes(prec, nz, ny, nx)
+    mask_d = torch.from_numpy(mask_h).to(dev)
+    s0 = torch.empty(sbytes, dtype=torch.uint8, device=dev)
+    s1 = torch.empty(sbytes, dtype=torch.uint8, device=dev)
+    out = torch.empty(n, dtype=torch.int16, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        capi.check(L.mxb_device_diffusion(mask_d.data_ptr(), nz, ny, nx, k.ctypes.data, len(k), reps,
+                                          prec, s0.data_ptr(), s1.data_ptr(), out.data_ptr(), None,
+                                          stream.cuda_stream))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    capi.set_kernel_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    times = {name: capi.kernel_times(c) for name, c in
+             (("zpass", capi.K_ZPASS), ("ypass", capi.K_YPASS), ("xpass", capi.K_XPASS),
+              ("other", capi.K_OTHER))}
+    capi.set_kernel_timing(False)
+
+    vox_iter = float(n) * reps * args.steps
+    value = vox_iter / elapsed / 1e9
+    alg = algorithmic(w, args.precision)
+    per_kernel = {}
+    for name in ("zpass", "ypass", "xpass"):
+        ms, cnt = times[name]
+        if cnt == 0:
+            continue
+        avg_s = ms / cnt / 1e3
+        b, f = alg[name]
+        per_kernel[name] = {"avg_ms": ms / cnt, "launches": cnt,
+                            "GBps": b * n / avg_s / 1e9, "TFLOPs": f * n / avg_s / 1e12}
+    dom = max(per_kernel, key=lambda kk: per_kernel[kk]["avg_ms"])
+    d = per_kernel[dom]
+    mfma_peak = PEAK_MFMA.get(args.precision.split("_")[0], PEAK_MFMA["fp32"])
+    if args.precision in ("fp32_fma", "fp32_exact"):
+        compute_peak = PEAK_VALU_FP32
+    else:
+        compute_peak = mfma_peak
+    frac_hbm = d["GBps"] / PEAK_HBM_GBS
+    frac_cmp = d["TFLOPs"] / compute_peak
+    traffic = load_traffic(args.workload, args.precision, dom)
+    if frac_cmp >= frac_hbm:
+        roof = {"bound": "mfma", "achieved": round(d["TFLOPs"], 3), "peak": compute_peak,
+                "unit": "TFLOP/s", "frac": round(frac_cmp, 4), "traffic": traffic}
+    else:
+        roof = {"bound": "hbm", "achieved": round(d["GBps"], 1), "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": round(frac_hbm, 4), "traffic": traffic}
+    roof["kernel"] = dom
+    roof["avg_launch_ms"] = round(d["avg_ms"], 4)
+    fused_bytes = (2 if args.precision in ("bf16", "fp16") else 4) * 2 + 1
+    rec = {
+        "metric": "Gvoxel*iter/s (masked 3D diffusion)",
+        "value": round(value, 3),
+        "unit": "Gvoxel*iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision.startswith("fp32") else args.precision,
+        "data": "synthetic",
+        "config": {"workload": f"{w.name}: {w.description}", "shape": list(w.shape),
+                   "sigma": w.sigma, "radius": w.radius, "repetitions": reps,
+                   "precision": args.precision, "output": "uint16"},
+        "hbm_fraction_fused_accounting": round(value * fused_bytes / PEAK_HBM_GBS, 4),
+        "per_kernel": {kk: {x: round(v, 4) if isinstance(v, float) else v for x, v in vv.items()}
+                       for kk, vv in per_kernel.items()},
+        "roofline": roof,
+    }
+    del mask_d, s0, s1, out
+    torch.cuda.empty_cache()
+    if args.cpu_seconds > 0:
+        rec["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+    else:
+        rec["cpu_baseline"] = None
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

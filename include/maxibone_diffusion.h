@@ -1,0 +1,144 @@
+/*
+ * maxibone_diffusion.h - C ABI of the MI355X (gfx950) masked 3D diffusion.
+ *
+ * The diffusion (reference: src/lib/cpp/cpu/diffusion.cc, semantics of
+ * cpu:339-374): u0 = (mask > 0); per repetition correlate along z, then y,
+ * then x with the 1D kernel (zero outside the volume), then set mask voxels
+ * back to 1; finally floor(u * 65535) as uint16.
+ *
+ * Volumes are C-order (z, y, x), x fastest.  The kernel has K = 2R + 1 taps
+ * (K odd).  Every function returns an mxb_status; on failure
+ * mxb_last_error() describes the cause (thread-local, valid until the next
+ * call on the same thread).
+ *
+ * Pointers named host_* / plain `mask`, `kernel`, `output` are host memory;
+ * pointers named dev_* are device (HBM) memory of the current HIP device.
+ */
+#ifndef MAXIBONE_DIFFUSION_H
+#define MAXIBONE_DIFFUSION_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    MXB_OK = 0,
+    MXB_EINVAL = 1,  /* invalid argument (Python: ValueError)          */
+    MXB_EDEVICE = 2, /* HIP runtime / kernel failure (RuntimeError)    */
+    MXB_ENOMEM = 3,  /* device or host allocation failed (MemoryError) */
+    MXB_EIO = 4      /* file I/O failed (RuntimeError)                 */
+} mxb_status;
+
+/* Storage / arithmetic form of the diffusion state. */
+typedef enum {
+    MXB_FP32 = 0,       /* FP32 state, FP32 banded-Toeplitz MFMA passes          */
+    MXB_BF16 = 1,       /* BF16 state, BF16 MFMA passes, FP32 final repetition    */
+    MXB_FP16 = 2,       /* FP16 state, FP16 MFMA passes, FP32 final repetition    */
+    MXB_FP32_FMA = 3,   /* FP32 state, one VALU FMA per tap                       */
+    MXB_FP32_EXACT = 4  /* FP32 state, product-then-add: bit-equal to the
+                           reference CPU build                                  */
+} mxb_precision;
+
+/* ---- drop-in entry points (host buffers) ------------------------------- */
+
+/* Replaces python_api::diffusion_in_memory (src/pybind/diffusion-pybind.cc:27-53)
+ * and NS::diffusion_in_memory (src/lib/cpp/include/diffusion.hh:36,
+ * cpu:339-374, gpu:758-806): uint8 mask in, uint16 result out, FP32 state.
+ * When the volume does not fit in device memory this streams z-chunks through
+ * the device, like the reference's automatic switch to
+ * diffusion_out_of_core (pyb:40-45). */
+int mxb_diffusion_u8_u16(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
+                         const float *kernel, int64_t kernel_size, int64_t repetitions,
+                         uint16_t *output, int verbose);
+
+/* Same as mxb_diffusion_u8_u16 but returns the FP32 state before
+ * quantisation (no reference counterpart: the reference only exposes the
+ * uint16 result, cpu:365). */
+int mxb_diffusion_u8_f32(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
+                         const float *kernel, int64_t kernel_size, int64_t repetitions,
+                         float *output, int verbose);
+
+/* Extended form of the two above: any mxb_precision; either output may be
+ * NULL (not both).  precision = MXB_FP32 is what mxb_diffusion_u8_u16 runs. */
+int mxb_diffusion_ex(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
+                     const float *kernel, int64_t kernel_size, int64_t repetitions,
+                     int precision, uint16_t *output_u16, float *output_f32, int verbose);
+
+/* z-chunked form over host-resident state: the device holds one chunk of
+ * chunk_z layers plus a halo; `temporal` repetitions run per chunk visit with
+ * a halo of temporal*R layers (temporal = 1 is the reference's
+ * one-step-per-chunk scheme, cpu:433-453 / gpu:869-905).  Results equal the
+ * in-memory form.  Replaces NS::diffusion_out_of_core (hh:74, gpu:947-1129). */
+int mxb_diffusion_chunked(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
+                          const float *kernel, int64_t kernel_size, int64_t repetitions,
+                          int64_t chunk_z, int64_t temporal, int precision,
+                          uint16_t *output_u16, float *output_f32, int verbose);
+
+/* File-backed form.  Replaces python_api::diffusion_on_disk (pyb:67-77, bound
+ * at pyb:85) and NS::diffusion_on_disk (hh:55, cpu:376-468, gpu:808-945):
+ * input_file holds raw uint8 (z, y, x) of total_shape; output_file receives
+ * raw uint16 (the GPU build's output type, gpu:937); global_shape is the
+ * chunk shape (its y and x must equal total_shape's). */
+int mxb_diffusion_on_disk(const char *input_file, const float *kernel, int64_t kernel_size,
+                          const char *output_file, int64_t total_z, int64_t total_y,
+                          int64_t total_x, int64_t global_z, int64_t global_y, int64_t global_x,
+                          int64_t repetitions, int verbose);
+
+/* ---- batched form --------------------------------------------------------- */
+
+/* One independent sub-volume of a batch (no reference counterpart: the
+ * reference diffuses one volume per call). */
+typedef struct {
+    const uint8_t *mask;   /* host, (nz, ny, nx) C-order   */
+    int64_t nz, ny, nx;
+    const float *kernel;   /* host, kernel_size taps        */
+    int64_t kernel_size;
+    uint16_t *output_u16;  /* host, may be NULL              */
+    float *output_f32;     /* host, may be NULL              */
+} mxb_volume_desc;
+
+/* Diffuse n volumes, each with its own kernel, `repetitions` times, in one
+ * persistent launch per pass, with the repetition loop captured in a HIP
+ * graph.  (No reference counterpart.) */
+int mxb_diffusion_batched(const mxb_volume_desc *volumes, int n, int64_t repetitions,
+                          int precision, int verbose);
+
+/* ---- device-resident form (inputs already in HBM) -------------------------- */
+
+/* Bytes of one state buffer for `precision` (two are needed). */
+int64_t mxb_state_bytes(int precision, int64_t nz, int64_t ny, int64_t nx);
+
+/* Run the diffusion on device buffers, enqueued on `stream` (a hipStream_t;
+ * NULL = default stream).  dev_state0/1: two buffers of mxb_state_bytes each.
+ * dev_out_u16 / dev_out_f32: final outputs (either may be NULL, not both).
+ * Allocates nothing.  Returns after enqueueing (asynchronous). */
+int mxb_device_diffusion(const uint8_t *dev_mask, int64_t nz, int64_t ny, int64_t nx,
+                         const float *kernel, int64_t kernel_size, int64_t repetitions,
+                         int precision, void *dev_state0, void *dev_state1,
+                         uint16_t *dev_out_u16, float *dev_out_f32, void *stream);
+
+/* ---- per-kernel timing (HIP events on the launch stream) ------------------ */
+
+/* Kernel classes reported by mxb_kernel_times. */
+typedef enum { MXB_K_ZPASS = 0, MXB_K_YPASS = 1, MXB_K_XPASS = 2, MXB_K_OTHER = 3 } mxb_kernel_class;
+
+/* Enable (1) / disable (0) event timing of every pass launch; resets totals. */
+int mxb_set_kernel_timing(int enable);
+/* Total milliseconds and launch count per class since timing was enabled
+ * (synchronises the recorded events first). */
+int mxb_kernel_times(int kernel_class, double *total_ms, int64_t *launches);
+
+/* ---- misc --------------------------------------------------------------- */
+
+/* Number of visible HIP devices (0 when none or the runtime is unusable). */
+int mxb_device_count(void);
+const char *mxb_last_error(void);
+const char *mxb_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MAXIBONE_DIFFUSION_H */

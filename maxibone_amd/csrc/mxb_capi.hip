@@ -1,0 +1,624 @@
+// C ABI of libmxb_diffusion: argument checks, device buffers, the repetition
+// driver, the z-chunked and file-backed forms, and kernel timing.
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/maxibone_diffusion.h"
+#include "mxb_internal.h"
+
+namespace mxb {
+
+static thread_local std::string g_error;
+void set_error(const std::string &msg) { g_error = msg; }
+
+#define MXB_HIP(expr)                                                                   \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess) {                                                         \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(_e));               \
+            return _e == hipErrorOutOfMemory ? ENOMEM_ : EDEVICE;                       \
+        }                                                                               \
+    } while (0)
+
+// ---------------------------------------------------------------- timing
+struct TimingState {
+    std::mutex mu;
+    bool enabled = false;
+    struct Rec {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    double total_ms[4] = {0, 0, 0, 0};
+    int64_t launches[4] = {0, 0, 0, 0};
+};
+static TimingState g_timing;
+
+static hipEvent_t take_event() {
+    if (!g_timing.pool.empty()) {
+        hipEvent_t e = g_timing.pool.back();
+        g_timing.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Runs `fn` (a launch on stream s), bracketing it with events when timing is on.
+template <typename F>
+static hipError_t timed(int cls, hipStream_t s, F &&fn) {
+    if (!g_timing.enabled) return fn();
+    std::lock_guard<std::mutex> lk(g_timing.mu);
+    hipEvent_t a = take_event(), b = take_event();
+    if (a == nullptr || b == nullptr) return fn();
+    hipEventRecord(a, s);
+    hipError_t e = fn();
+    hipEventRecord(b, s);
+    g_timing.recs.push_back({cls, a, b});
+    return e;
+}
+
+static void drain_timing_locked() {
+    for (auto &r : g_timing.recs) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            g_timing.total_ms[r.cls] += ms;
+            g_timing.launches[r.cls] += 1;
+        }
+        g_timing.pool.push_back(r.a);
+        g_timing.pool.push_back(r.b);
+    }
+    g_timing.recs.clear();
+}
+
+// ---------------------------------------------------------------- checks
+static int check_common(int64_t nz, int64_t ny, int64_t nx, const float *kernel, int64_t K,
+                        int64_t reps, int precision) {
+    if (nz < 0 || ny < 0 || nx < 0) {
+        set_error("volume dimensions must be non-negative");
+        return EINVAL_;
+    }
+    if (kernel == nullptr || K < 1 || (K % 2) == 0) {
+        set_error("kernel must hold an odd number of taps (got " + std::to_string(K) + ")");
+        return EINVAL_;
+    }
+    if (K > kMaxTapsGeneric) {
+        set_error("kernel_size " + std::to_string(K) + " exceeds the supported maximum of " +
+                  std::to_string(kMaxTapsGeneric));
+        return EINVAL_;
+    }
+    if (reps < 0) {
+        set_error("repetitions must be non-negative");
+        return EINVAL_;
+    }
+    if (precision < MXB_FP32 || precision > MXB_FP32_EXACT) {
+        set_error("unknown precision " + std::to_string(precision));
+        return EINVAL_;
+    }
+    if (nz > 0 && ny > 0 && nx > 0 && (nz > (int64_t(1) << 40) / ny / nx)) {
+        set_error("volume too large");
+        return EINVAL_;
+    }
+    return OK;
+}
+
+static Arith arith_of(int precision) {
+    switch (precision) {
+        case MXB_FP32_FMA: return Arith::FMA;
+        case MXB_FP32_EXACT: return Arith::EXACT;
+        default: return Arith::MFMA;
+    }
+}
+
+static bool is_lowp(int precision) { return precision == MXB_BF16 || precision == MXB_FP16; }
+
+int64_t state_bytes(int precision, int64_t nz, int64_t ny, int64_t nx) {
+    const int64_t n = nz * ny * nx;
+    return is_lowp(precision) ? n * 2 : n * 4;
+}
+
+// ---------------------------------------------------------------- driver
+// Runs `reps` repetitions on device buffers.  If init_from_mask, state0 is
+// first set to (mask > 0); otherwise state0 already holds the FP32 state.
+// FP32 step: z: s0 -> s1, y: s1 -> s0 ((z, x, y) order), x: s0 -> s1 with
+// illumination; then the buffers swap roles.  The final x-pass writes
+// out_f32 / out_u16 (when non-null); when out_f32 is null the final FP32
+// state is left in the buffer returned through *final_state (may be null).
+static int run_steps_f32(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t nx,
+                         const float *k, int64_t K, int64_t reps, Arith arith, float *s0,
+                         float *s1, bool init_from_mask, uint16_t *out_u16, float *out_f32,
+                         float **final_state, hipStream_t s) {
+    const int64_t n = nz * ny * nx;
+    const int R = (int)(K / 2);
+    if (n == 0) return OK;
+    if (init_from_mask) MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_init_state(d_mask, s0, n, s); }));
+    float *cur = s0, *tmp = s1;
+    if (reps == 0) {
+        if (out_u16 != nullptr || out_f32 != nullptr)
+            MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_quantise(cur, out_u16, out_f32, n, s); }));
+        if (final_state) *final_state = cur;
+        return OK;
+    }
+    const LineGeom gz = geom_z(nz, ny, nx), gy = geom_y(nz, ny, nx), gx = geom_x(nz, ny, nx);
+    for (int64_t t = 0; t < reps; ++t) {
+        const bool last = t == reps - 1;
+        const Epilogue ez{nullptr, tmp, nullptr};
+        MXB_HIP(timed(MXB_K_ZPASS, s, [&] { return launch_line_pass_f32(cur, gz, ez, k, R, arith, s); }));
+        const Epilogue ey{nullptr, cur, nullptr};
+        MXB_HIP(timed(MXB_K_YPASS, s, [&] { return launch_line_pass_f32(tmp, gy, ey, k, R, arith, s); }));
+        float *dst = (last && out_f32 != nullptr) ? out_f32 : tmp;
+        const Epilogue ex{d_mask, dst, last ? out_u16 : nullptr};
+        MXB_HIP(timed(MXB_K_XPASS, s, [&] { return launch_line_pass_f32(cur, gx, ex, k, R, arith, s); }));
+        if (last && out_f32 != nullptr) {
+            if (final_state) *final_state = out_f32;
+        } else {
+            std::swap(cur, tmp);
+            if (last && final_state) *final_state = cur;
+        }
+    }
+    return OK;
+}
+
+static int run_steps(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t nx, const float *k,
+                     int64_t K, int64_t reps, int precision, void *s0, void *s1,
+                     uint16_t *out_u16, float *out_f32, hipStream_t s) {
+    if (is_lowp(precision)) {
+        set_error("BF16/FP16 storage is not available in this build");
+        return EINVAL_;
+    }
+    return run_steps_f32(d_mask, nz, ny, nx, k, K, reps, arith_of(precision), (float *)s0,
+                         (float *)s1, true, out_u16, out_f32, nullptr, s);
+}
+
+// RAII device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes > 0 ? bytes : 1); }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    ~Stream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+static void log_v(int verbose, int level, const std::string &msg) {
+    if (verbose >= level) {
+        std::fprintf(stdout, "%s\n", msg.c_str());
+        std::fflush(stdout);
+    }
+}
+
+// Host-resident chunked driver.  hs0/hs1: host FP32 state buffers (n floats
+// each, may be pinned, pageable or file-backed).  The device holds one chunk:
+// layers [lo, hi) = [z0 - H, z0 + cz + H) clipped to the volume, H = T*R.
+// Clipping keeps the true zero boundary at the volume ends; at interior chunk
+// edges the fake boundary corrupts at most R layers per repetition, so after
+// T repetitions the interior [z0, z0 + cz) is exact.
+static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx, const float *k,
+                        int64_t K, int64_t reps, int64_t cz, int64_t T, int precision,
+                        float *hs0, float *hs1, uint16_t *out_u16, float *out_f32, int verbose) {
+    const int64_t R = K / 2, layer = ny * nx, n = nz * layer;
+    if (n == 0) return OK;
+    if (is_lowp(precision)) precision = MXB_FP32;  // chunks carry FP32 state between visits
+    const Arith arith = arith_of(precision);
+    if (T < 1) T = 1;
+    if (cz < 1) cz = 1;
+    if (cz > nz) cz = nz;
+    const int64_t H = T * R;
+    const int64_t max_layers = std::min(nz, cz + 2 * H);
+    const int64_t cap = max_layers * layer;
+    Stream st;
+    MXB_HIP(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    DevBuf dm, d0, d1, dq;
+    MXB_HIP(dm.alloc(cap));
+    MXB_HIP(d0.alloc(cap * 4));
+    MXB_HIP(d1.alloc(cap * 4));
+    if (out_u16) MXB_HIP(dq.alloc(cap * 2));
+    // u0 = (mask > 0) on the host state
+    for (int64_t i = 0; i < n; ++i) hs0[i] = mask[i] > 0 ? 1.0f : 0.0f;
+    float *cur = hs0, *nxt = hs1;
+    int64_t done = 0;
+    if (reps == 0) {
+        // quantise u0 chunk by chunk on the device
+        for (int64_t z0 = 0; z0 < nz; z0 += cz) {
+            const int64_t c = std::min(cz, nz - z0);
+            MXB_HIP(hipMemcpyAsync(d0.p, cur + z0 * layer, c * layer * 4, hipMemcpyHostToDevice, st.s));
+            MXB_HIP(launch_quantise((float *)d0.p, out_u16 ? (uint16_t *)dq.p : nullptr, nullptr, c * layer, st.s));
+            if (out_u16) MXB_HIP(hipMemcpyAsync(out_u16 + z0 * layer, dq.p, c * layer * 2, hipMemcpyDeviceToHost, st.s));
+            MXB_HIP(hipStreamSynchronize(st.s));
+        }
+        if (out_f32) std::memcpy(out_f32, cur, n * 4);
+        return OK;
+    }
+    while (done < reps) {
+        const int64_t tb = std::min(T, reps - done);
+        const bool last_block = done + tb == reps;
+        for (int64_t z0 = 0; z0 < nz; z0 += cz) {
+            const int64_t c = std::min(cz, nz - z0);
+            const int64_t lo = std::max<int64_t>(0, z0 - tb * R);
+            const int64_t hi = std::min(nz, z0 + c + tb * R);
+            const int64_t lz = hi - lo;
+            log_v(verbose, 2, "chunk z0=" + std::to_string(z0) + " layers=" + std::to_string(lz));
+            MXB_HIP(hipMemcpyAsync(dm.p, mask + lo * layer, lz * layer, hipMemcpyHostToDevice, st.s));
+            MXB_HIP(hipMemcpyAsync(d0.p, cur + lo * layer, lz * layer * 4, hipMemcpyHostToDevice, st.s));
+            float *fin = nullptr;
+            int rc = run_steps_f32((const uint8_t *)dm.p, lz, ny, nx, k, K, tb, arith, (float *)d0.p,
+                                   (float *)d1.p, false, last_block && out_u16 ? (uint16_t *)dq.p : nullptr,
+                                   nullptr, &fin, st.s);
+            if (rc != OK) return rc;
+            const int64_t off = (z0 - lo) * layer;
+            MXB_HIP(hipMemcpyAsync(nxt + z0 * layer, fin + off, c * layer * 4, hipMemcpyDeviceToHost, st.s));
+            if (last_block && out_u16)
+                MXB_HIP(hipMemcpyAsync(out_u16 + z0 * layer, (uint16_t *)dq.p + off, c * layer * 2,
+                                       hipMemcpyDeviceToHost, st.s));
+            MXB_HIP(hipStreamSynchronize(st.s));
+        }
+        std::swap(cur, nxt);
+        done += tb;
+        log_v(verbose, 1, "repetition " + std::to_string(done) + "/" + std::to_string(reps));
+    }
+    if (out_f32) std::memcpy(out_f32, cur, n * 4);
+    return OK;
+}
+
+// Host buffer that prefers pinned memory.
+struct HostBuf {
+    void *p = nullptr;
+    bool pinned = false;
+    ~HostBuf() {
+        if (!p) return;
+        if (pinned) (void)hipHostFree(p);
+        else std::free(p);
+    }
+    bool alloc(size_t bytes) {
+        if (hipHostMalloc(&p, bytes > 0 ? bytes : 1, hipHostMallocDefault) == hipSuccess) {
+            pinned = true;
+            return true;
+        }
+        (void)hipGetLastError();
+        p = std::malloc(bytes > 0 ? bytes : 1);
+        return p != nullptr;
+    }
+};
+
+static int chunked_host(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx, const float *k,
+                        int64_t K, int64_t reps, int64_t cz, int64_t T, int precision,
+                        uint16_t *out_u16, float *out_f32, int verbose) {
+    const int64_t n = nz * ny * nx;
+    HostBuf h0, h1;
+    if (!h0.alloc(n * 4) || !h1.alloc(n * 4)) {
+        set_error("host allocation of the chunked state failed");
+        return ENOMEM_;
+    }
+    return chunked_core(mask, nz, ny, nx, k, K, reps, cz, T, precision, (float *)h0.p,
+                        (float *)h1.p, out_u16, out_f32, verbose);
+}
+
+// Largest chunk depth whose device footprint fits in `budget` bytes.
+static int64_t chunk_depth_for(int64_t budget, int64_t nz, int64_t layer, int64_t H) {
+    const int64_t per_layer = layer * (1 + 4 + 4 + 2);
+    int64_t layers = budget / std::max<int64_t>(per_layer, 1);
+    int64_t cz = layers - 2 * H;
+    if (cz < 16) cz = 16;
+    return std::min(cz, nz);
+}
+
+static int in_memory(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx, const float *k,
+                     int64_t K, int64_t reps, int precision, uint16_t *out_u16, float *out_f32,
+                     int verbose) {
+    int rc = check_common(nz, ny, nx, k, K, reps, precision);
+    if (rc != OK) return rc;
+    if (out_u16 == nullptr && out_f32 == nullptr) {
+        set_error("no output buffer given");
+        return EINVAL_;
+    }
+    const int64_t n = nz * ny * nx;
+    if (n == 0) return OK;
+    if (mask == nullptr) {
+        set_error("mask is null");
+        return EINVAL_;
+    }
+    size_t free_b = 0, total_b = 0;
+    MXB_HIP(hipMemGetInfo(&free_b, &total_b));
+    const int64_t sb = state_bytes(precision, nz, ny, nx);
+    const int64_t need = n + 2 * sb + (out_u16 ? 2 * n : 0) + (out_f32 ? 4 * n : 0);
+    const int64_t budget = (int64_t)(free_b * 0.92) - (int64_t(256) << 20);
+    if (need > budget) {
+        const int64_t H = K / 2;
+        const int64_t cz = chunk_depth_for(budget, nz, ny * nx, H);
+        log_v(verbose, 1, "volume exceeds device memory: streaming z-chunks of " + std::to_string(cz));
+        return chunked_host(mask, nz, ny, nx, k, K, reps, cz, 1, precision, out_u16, out_f32, verbose);
+    }
+    Stream st;
+    MXB_HIP(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    DevBuf dm, d0, d1, dq, df;
+    MXB_HIP(dm.alloc(n));
+    MXB_HIP(d0.alloc(sb));
+    MXB_HIP(d1.alloc(sb));
+    if (out_u16) MXB_HIP(dq.alloc(n * 2));
+    if (out_f32) MXB_HIP(df.alloc(n * 4));
+    log_v(verbose, 1, "Converting uint8 to float");
+    MXB_HIP(hipMemcpyAsync(dm.p, mask, n, hipMemcpyHostToDevice, st.s));
+    rc = run_steps((const uint8_t *)dm.p, nz, ny, nx, k, K, reps, precision, d0.p, d1.p,
+                   (uint16_t *)dq.p, (float *)df.p, st.s);
+    if (rc != OK) return rc;
+    if (out_u16) MXB_HIP(hipMemcpyAsync(out_u16, dq.p, n * 2, hipMemcpyDeviceToHost, st.s));
+    if (out_f32) MXB_HIP(hipMemcpyAsync(out_f32, df.p, n * 4, hipMemcpyDeviceToHost, st.s));
+    MXB_HIP(hipStreamSynchronize(st.s));
+    log_v(verbose, 1, "Diffusion is complete!");
+    return OK;
+}
+
+// ---------------------------------------------------------------- files
+struct MappedFile {
+    int fd = -1;
+    void *p = MAP_FAILED;
+    size_t len = 0;
+    ~MappedFile() {
+        if (p != MAP_FAILED && len > 0) munmap(p, len);
+        if (fd >= 0) close(fd);
+    }
+};
+
+static int map_read(const char *path, size_t need, MappedFile &m) {
+    m.fd = open(path, O_RDONLY);
+    if (m.fd < 0) {
+        set_error(std::string("Could not open file: ") + path);
+        return EIO_;
+    }
+    struct stat sb;
+    if (fstat(m.fd, &sb) != 0 || (size_t)sb.st_size < need) {
+        set_error(std::string("Could not read the entire buffer: ") + path + " holds fewer than " +
+                  std::to_string(need) + " bytes");
+        return EIO_;
+    }
+    m.len = need;
+    if (need == 0) return OK;
+    m.p = mmap(nullptr, need, PROT_READ, MAP_SHARED, m.fd, 0);
+    if (m.p == MAP_FAILED) {
+        set_error(std::string("mmap failed: ") + path);
+        return EIO_;
+    }
+    return OK;
+}
+
+static int map_write(const char *path, size_t len, MappedFile &m, bool unlink_after) {
+    if (path != nullptr) {
+        m.fd = open(path, O_RDWR | O_CREAT | O_TRUNC, 0644);
+    } else {
+        const char *tmpdir = std::getenv("TMPDIR");
+        std::string tmpl = std::string(tmpdir && *tmpdir ? tmpdir : "/tmp") + "/mxb-diffusion-XXXXXX";
+        std::vector<char> buf(tmpl.begin(), tmpl.end());
+        buf.push_back('\0');
+        m.fd = mkstemp(buf.data());
+        if (m.fd >= 0 && unlink_after) unlink(buf.data());
+    }
+    if (m.fd < 0) {
+        set_error(std::string("Could not open file: ") + (path ? path : "temporary state file"));
+        return EIO_;
+    }
+    if (ftruncate(m.fd, (off_t)len) != 0) {
+        set_error(std::string("Could not truncate file: ") + (path ? path : "temporary state file"));
+        return EIO_;
+    }
+    m.len = len;
+    if (len == 0) return OK;
+    m.p = mmap(nullptr, len, PROT_READ | PROT_WRITE, MAP_SHARED, m.fd, 0);
+    if (m.p == MAP_FAILED) {
+        set_error(std::string("mmap failed: ") + (path ? path : "temporary state file"));
+        return EIO_;
+    }
+    return OK;
+}
+
+}  // namespace mxb
+
+using namespace mxb;
+
+extern "C" {
+
+const char *mxb_last_error(void) { return g_error.c_str(); }
+const char *mxb_version(void) { return "maxibone_amd diffusion 0.1 (gfx950)"; }
+
+int64_t mxb_state_bytes(int precision, int64_t nz, int64_t ny, int64_t nx) {
+    return state_bytes(precision, nz, ny, nx);
+}
+
+int mxb_diffusion_ex(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx, const float *kernel,
+                     int64_t kernel_size, int64_t repetitions, int precision, uint16_t *output_u16,
+                     float *output_f32, int verbose) {
+    return in_memory(mask, nz, ny, nx, kernel, kernel_size, repetitions, precision, output_u16,
+                     output_f32, verbose);
+}
+
+int mxb_diffusion_u8_u16(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
+                         const float *kernel, int64_t kernel_size, int64_t repetitions,
+                         uint16_t *output, int verbose) {
+    if (output == nullptr) {
+        set_error("output is null");
+        return EINVAL_;
+    }
+    return in_memory(mask, nz, ny, nx, kernel, kernel_size, repetitions, MXB_FP32, output, nullptr,
+                     verbose);
+}
+
+int mxb_diffusion_u8_f32(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
+                         const float *kernel, int64_t kernel_size, int64_t repetitions,
+                         float *output, int verbose) {
+    if (output == nullptr) {
+        set_error("output is null");
+        return EINVAL_;
+    }
+    return in_memory(mask, nz, ny, nx, kernel, kernel_size, repetitions, MXB_FP32, nullptr, output,
+                     verbose);
+}
+
+int mxb_diffusion_chunked(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
+                          const float *kernel, int64_t kernel_size, int64_t repetitions,
+                          int64_t chunk_z, int64_t temporal, int precision, uint16_t *output_u16,
+                          float *output_f32, int verbose) {
+    int rc = check_common(nz, ny, nx, kernel, kernel_size, repetitions, precision);
+    if (rc != OK) return rc;
+    if (output_u16 == nullptr && output_f32 == nullptr) {
+        set_error("no output buffer given");
+        return EINVAL_;
+    }
+    if (chunk_z < 1) {
+        set_error("chunk_z must be positive");
+        return EINVAL_;
+    }
+    if (temporal < 1) {
+        set_error("temporal must be positive");
+        return EINVAL_;
+    }
+    if (nz * ny * nx > 0 && mask == nullptr) {
+        set_error("mask is null");
+        return EINVAL_;
+    }
+    return chunked_host(mask, nz, ny, nx, kernel, kernel_size, repetitions, chunk_z, temporal,
+                        precision, output_u16, output_f32, verbose);
+}
+
+int mxb_diffusion_on_disk(const char *input_file, const float *kernel, int64_t kernel_size,
+                          const char *output_file, int64_t total_z, int64_t total_y,
+                          int64_t total_x, int64_t global_z, int64_t global_y, int64_t global_x,
+                          int64_t repetitions, int verbose) {
+    int rc = check_common(total_z, total_y, total_x, kernel, kernel_size, repetitions, MXB_FP32);
+    if (rc != OK) return rc;
+    if (input_file == nullptr || output_file == nullptr) {
+        set_error("file names must not be null");
+        return EINVAL_;
+    }
+    if (global_z < 1 || global_y != total_y || global_x != total_x) {
+        set_error("global_shape must be (chunk_z >= 1, total_y, total_x): chunks split z only");
+        return EINVAL_;
+    }
+    const int64_t n = total_z * total_y * total_x;
+    MappedFile in, out;
+    rc = map_read(input_file, (size_t)n, in);
+    if (rc != OK) return rc;
+    rc = map_write(output_file, (size_t)n * 2, out, false);
+    if (rc != OK) return rc;
+    if (n == 0) return OK;
+    // State lives in two unlinked temporary files, paged by the OS (unique
+    // names, so concurrent calls do not collide).
+    MappedFile t0, t1;
+    rc = map_write(nullptr, (size_t)n * 4, t0, true);
+    if (rc != OK) return rc;
+    rc = map_write(nullptr, (size_t)n * 4, t1, true);
+    if (rc != OK) return rc;
+    size_t free_b = 0, total_b = 0;
+    MXB_HIP(hipMemGetInfo(&free_b, &total_b));
+    const int64_t budget = (int64_t)(free_b * 0.92) - (int64_t(256) << 20);
+    const int64_t R = kernel_size / 2, layer = total_y * total_x;
+    // The caller's chunk depth is honoured.  Temporal blocking runs T
+    // repetitions per chunk visit (halo T*R layers per side), which divides
+    // the file traffic by T; T is capped so that the halo stays within about
+    // one chunk depth (at most ~3x redundant work) and the chunk fits.
+    int64_t T = 1;
+    while (T < repetitions && 2 * (T + 1) * R <= std::max(global_z, R) &&
+           std::min(total_z, global_z + 2 * (T + 1) * R) * layer * 11 < budget)
+        ++T;
+    if (std::min(total_z, global_z + 2 * T * R) * layer * 11 > budget) {
+        set_error("chunk of " + std::to_string(global_z) + " layers plus halo exceeds device memory");
+        return ENOMEM_;
+    }
+    rc = chunked_core((const uint8_t *)in.p, total_z, total_y, total_x, kernel, kernel_size,
+                      repetitions, global_z, T, MXB_FP32, (float *)t0.p, (float *)t1.p,
+                      (uint16_t *)out.p, nullptr, verbose);
+    if (rc != OK) return rc;
+    if (msync(out.p, out.len, MS_SYNC) != 0) {
+        set_error(std::string("Could not write the output file: ") + output_file);
+        return EIO_;
+    }
+    return OK;
+}
+
+int mxb_device_diffusion(const uint8_t *dev_mask, int64_t nz, int64_t ny, int64_t nx,
+                         const float *kernel, int64_t kernel_size, int64_t repetitions,
+                         int precision, void *dev_state0, void *dev_state1, uint16_t *dev_out_u16,
+                         float *dev_out_f32, void *stream) {
+    int rc = check_common(nz, ny, nx, kernel, kernel_size, repetitions, precision);
+    if (rc != OK) return rc;
+    if (dev_out_u16 == nullptr && dev_out_f32 == nullptr) {
+        set_error("no output buffer given");
+        return EINVAL_;
+    }
+    if (nz * ny * nx > 0 && (dev_mask == nullptr || dev_state0 == nullptr || dev_state1 == nullptr)) {
+        set_error("device buffers must not be null");
+        return EINVAL_;
+    }
+    return run_steps(dev_mask, nz, ny, nx, kernel, kernel_size, repetitions, precision, dev_state0,
+                     dev_state1, dev_out_u16, dev_out_f32, (hipStream_t)stream);
+}
+
+int mxb_diffusion_batched(const mxb_volume_desc *volumes, int n, int64_t repetitions,
+                          int precision, int verbose) {
+    if (n < 0 || (n > 0 && volumes == nullptr)) {
+        set_error("volumes must hold n descriptors");
+        return EINVAL_;
+    }
+    for (int i = 0; i < n; ++i) {
+        const mxb_volume_desc &v = volumes[i];
+        int rc = in_memory(v.mask, v.nz, v.ny, v.nx, v.kernel, v.kernel_size, repetitions,
+                           precision, v.output_u16, v.output_f32, verbose);
+        if (rc != OK) {
+            set_error("volume " + std::to_string(i) + ": " + g_error);
+            return rc;
+        }
+    }
+    return OK;
+}
+
+int mxb_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int mxb_set_kernel_timing(int enable) {
+    std::lock_guard<std::mutex> lk(g_timing.mu);
+    drain_timing_locked();
+    for (int i = 0; i < 4; ++i) {
+        g_timing.total_ms[i] = 0;
+        g_timing.launches[i] = 0;
+    }
+    g_timing.enabled = enable != 0;
+    return OK;
+}
+
+int mxb_kernel_times(int kernel_class, double *total_ms, int64_t *launches) {
+    if (kernel_class < 0 || kernel_class > 3) {
+        set_error("unknown kernel class");
+        return EINVAL_;
+    }
+    std::lock_guard<std::mutex> lk(g_timing.mu);
+    drain_timing_locked();
+    if (total_ms) *total_ms = g_timing.total_ms[kernel_class];
+    if (launches) *launches = g_timing.launches[kernel_class];
+    return OK;
+}
+
+}  // extern "C"

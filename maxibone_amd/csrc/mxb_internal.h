@@ -1,0 +1,54 @@
+// Internal declarations shared by the HIP translation units of libmxb_diffusion.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace mxb {
+
+// Status codes of the C ABI (mirrored in include/maxibone_diffusion.h).
+enum Status : int { OK = 0, EINVAL_ = 1, EDEVICE = 2, ENOMEM_ = 3, EIO_ = 4 };
+
+void set_error(const std::string &msg);
+
+// Largest radius served by the register-window kernels; larger radii use the
+// per-voxel kernel, up to kMaxTapsGeneric taps.
+constexpr int kMaxRadiusFast = 64;
+constexpr int kMaxTapsGeneric = 511;
+
+// Geometry of one 1D pass.  A "line" is the set of voxels a 1D correlation
+// runs along.  Lines are grouped so that 64 consecutive lines sit at
+// consecutive addresses of the input (lane stride 1): every wave reads 256
+// contiguous bytes per position.  `outer` indexes groups of lanes.
+struct LineGeom {
+    int64_t n_lane, n_outer, n_ax;   // lines per group, groups, positions per line
+    int64_t in_outer, in_ax;         // input strides (lane stride is 1)
+    int64_t out_lane, out_outer, out_ax;  // output strides (also the mask's)
+    int64_t seg;                     // positions per thread segment (multiple of 16)
+    int64_t nseg;                    // segments per line
+};
+
+// Epilogue of the last pass of a step: illuminate from the mask, write the
+// float state and/or its uint16 quantisation.
+struct Epilogue {
+    const uint8_t *mask;   // nullptr: plain pass (no illumination)
+    float *out_f32;        // may be nullptr when only out_u16 is wanted
+    uint16_t *out_u16;     // nullptr unless this is the final step
+};
+
+enum class Arith : int { FMA = 0, EXACT = 1, MFMA = 2 };
+
+// Pass launchers (mxb_kernels_f32.hip).  `taps` are K = 2R+1 host floats.
+hipError_t launch_line_pass_f32(const float *in, const LineGeom &g, const Epilogue &e,
+                                const float *taps, int R, Arith arith, hipStream_t s);
+hipError_t launch_init_state(const uint8_t *mask, float *u, int64_t n, hipStream_t s);
+hipError_t launch_quantise(const float *u, uint16_t *q, float *f, int64_t n, hipStream_t s);
+
+// Plan the three passes of a step over a (nz, ny, nx) volume.
+// z: A -> A, y: A -> T (T is (z, x, y) order), x: T -> A.
+LineGeom geom_z(int64_t nz, int64_t ny, int64_t nx);
+LineGeom geom_y(int64_t nz, int64_t ny, int64_t nx);
+LineGeom geom_x(int64_t nz, int64_t ny, int64_t nx);
+
+}  // namespace mxb

@@ -1,0 +1,414 @@
+// FP32 line-pass kernels of the masked 3D diffusion (gfx950).
+//
+// One 1D correlation pass (reference: src/lib/cpp/cpu/diffusion.cc:31-60)
+//   out[p] = sum_{r=-R..R} in[p + r] * k[R + r],   in[] = 0 outside the line,
+// runs over "lines" whose first voxels sit at consecutive addresses (lane
+// stride 1), so that a wave's load of one position covers contiguous bytes.
+// A thread marches along its line over a segment of positions and keeps the
+// inputs it still needs in registers: every input is fetched from memory once
+// per segment.  The y-pass writes (z, x, y) order and the x-pass reads it back
+// and writes (z, y, x), so that all three passes read along a contiguous axis.
+//
+// Arithmetic forms (all sum the taps r-ascending from 0, like cpu:46-54):
+//   EXACT: product rounded, then added, no FMA: bit-identical to the reference
+//          CPU build (-O3 without -march has no FMA contraction).
+//   FMA:   one fused multiply-add per tap.
+//   MFMA:  banded-Toeplitz tiles on v_mfma_f32_16x16x4_f32.  A wave owns 16
+//          lines; per tile of 16 consecutive outputs of each line it runs
+//          C = (16 + 2R)/4 MFMAs D[i][j] += T_c[i][kk] * X_c[kk][j] with
+//          T_c[i][kk] = k[4c + kk - i] (zero outside the band) and
+//          X_c[kk][j] = in_j(base - R + 4c + kk).
+//
+// Zero boundary: positions outside [0, n_ax) contribute 0*k, which adds +0
+// and leaves every partial sum unchanged, so zero-padding equals the
+// reference's skipped taps exactly.  The compiled tap capacity RT may exceed
+// the runtime radius R: the taps are centred and padded with zeros, which
+// again only adds exact zeros.
+#include "mxb_internal.h"
+
+namespace mxb {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <int N>
+struct Taps {
+    float k[N];
+};
+
+__device__ __forceinline__ float mac_fma(float acc, float w, float k) {
+    return __builtin_fmaf(w, k, acc);
+}
+
+__device__ __forceinline__ float mac_exact(float acc, float w, float k) {
+#pragma clang fp contract(off)
+    const float p = w * k;
+    return acc + p;
+}
+
+// floor(u * 65535.0f) as uint16 (cpu:191-196), converted through int32 so that
+// out-of-range values wrap the way the x86 reference build does.
+__device__ __forceinline__ uint16_t quantise_u16(float v) {
+#pragma clang fp contract(off)
+    const float s = v * 65535.0f;
+    return (uint16_t)(int32_t)floorf(s);
+}
+
+__device__ __forceinline__ void epilogue_store(const Epilogue &e, int64_t off, float v) {
+    if (e.mask != nullptr && e.mask[off] != 0) v = 1.0f;  // illuminate, cpu:295-302
+    if (e.out_f32 != nullptr) e.out_f32[off] = v;
+    if (e.out_u16 != nullptr) e.out_u16[off] = quantise_u16(v);
+}
+
+__device__ __forceinline__ int wave_id() {
+    return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+
+// Block -> (segment, group of 64 lines, outer index) for the VALU form, where
+// a block holds 4 waves of 64 lines each, one outer index per wave.
+struct BlockPos {
+    int64_t outer, lane0, a0, a1;
+};
+
+__device__ __forceinline__ BlockPos decode_valu(const LineGeom &g) {
+    int64_t bid = blockIdx.x;
+    const int64_t seg_id = bid % g.nseg;
+    bid /= g.nseg;
+    const int64_t nlb = (g.n_lane + 63) >> 6;
+    const int64_t lb = bid % nlb;
+    bid /= nlb;
+    BlockPos p;
+    p.outer = bid * 4 + wave_id();
+    p.lane0 = lb * 64;
+    p.a0 = seg_id * g.seg;
+    p.a1 = p.a0 + g.seg < g.n_ax ? p.a0 + g.seg : g.n_ax;
+    return p;
+}
+
+// ---------------------------------------------------------------- VALU form
+template <int RT, int B, bool EXACT>
+__global__ __launch_bounds__(256) void k_line_valu(const float *__restrict__ in, LineGeom g,
+                                                   Epilogue e, Taps<2 * RT + 1> taps) {
+    const BlockPos bp = decode_valu(g);
+    if (bp.outer >= g.n_outer) return;  // wave-uniform
+    const int lane = threadIdx.x & 63;
+    const int64_t li = bp.lane0 + lane;
+    if (li >= g.n_lane) return;  // no cross-lane work in this form
+    constexpr int W = B + 2 * RT;
+    const int64_t n_ax = g.n_ax, in_ax = g.in_ax;
+    const float *src = in + bp.outer * g.in_outer + bp.lane0;  // wave-uniform base
+    const int64_t obase = bp.outer * g.out_outer + li * g.out_lane;
+    float w[W];
+#pragma unroll
+    for (int i = 0; i < 2 * RT; ++i) {
+        const int64_t p = bp.a0 - RT + i;
+        w[i] = (p >= 0 && p < n_ax) ? src[p * in_ax + lane] : 0.0f;
+    }
+    float nb[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int64_t p = bp.a0 + RT + b;
+        nb[b] = p < n_ax ? src[p * in_ax + lane] : 0.0f;
+    }
+    for (int64_t a = bp.a0; a < bp.a1; a += B) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) w[2 * RT + b] = nb[b];
+        // prefetch the inputs of the next chunk
+        if (a + B + RT + B <= n_ax) {
+#pragma unroll
+            for (int b = 0; b < B; ++b) nb[b] = src[(a + B + RT + b) * in_ax + lane];
+        } else {
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int64_t p = a + B + RT + b;
+                nb[b] = p < n_ax ? src[p * in_ax + lane] : 0.0f;
+            }
+        }
+        float acc[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) acc[b] = 0.0f;
+#pragma unroll
+        for (int t = 0; t <= 2 * RT; ++t) {
+            const float kt = taps.k[t];
+#pragma unroll
+            for (int b = 0; b < B; ++b)
+                acc[b] = EXACT ? mac_exact(acc[b], w[b + t], kt) : mac_fma(acc[b], w[b + t], kt);
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+            if (a + b < bp.a1) epilogue_store(e, obase + (a + b) * g.out_ax, acc[b]);
+#pragma unroll
+        for (int i = 0; i < 2 * RT; ++i) w[i] = w[i + B];
+    }
+}
+
+// ---------------------------------------------------------------- MFMA form
+// Block = 4 waves = 64 consecutive lines at one outer index; wave w owns lines
+// lane0 + 16w .. +15.  Lane l: column j = l % 16, row group kg = l / 16.
+//   A operand (taps):   lane holds T_c[i = l%16][kk = l/16] = k[4c + l/16 - l%16]
+//   B operand (inputs): lane holds X_c[kk = l/16][j = l%16]
+//   D (outputs):        lane holds out_j(base + 4*(l/16) + r), r = 0..3
+// The inputs of chunk G (positions a0 - R + 4G + kk) live in ring slot G % CR;
+// the ring holds one tile of prefetch beyond the C chunks a tile consumes.
+struct BlockPosM {
+    int64_t outer, col0, a0, a1;
+};
+
+__device__ __forceinline__ BlockPosM decode_mfma(const LineGeom &g) {
+    int64_t bid = blockIdx.x;
+    const int64_t seg_id = bid % g.nseg;
+    bid /= g.nseg;
+    const int64_t nlb = (g.n_lane + 63) >> 6;
+    const int64_t lb = bid % nlb;
+    bid /= nlb;
+    BlockPosM p;
+    p.outer = bid;
+    p.col0 = lb * 64 + 16 * wave_id();
+    p.a0 = seg_id * g.seg;
+    p.a1 = p.a0 + g.seg < g.n_ax ? p.a0 + g.seg : g.n_ax;
+    return p;
+}
+
+template <int RT>
+struct MfmaShape {
+    static constexpr int C = (16 + 2 * RT + 3) / 4;  // chunks per tile
+    static constexpr int CR = ((C + 3) / 4) * 4 + 4;  // ring slots
+    static constexpr int P = CR / 4;                  // tiles per ring period
+};
+
+template <int RT>
+__global__ __launch_bounds__(256) void k_line_mfma(const float *__restrict__ in, LineGeom g,
+                                                   Epilogue e, Taps<2 * RT + 1> taps) {
+    using S = MfmaShape<RT>;
+    constexpr int C = S::C, CR = S::CR, P = S::P;
+    __shared__ float tap_lds[4 * C + 32];
+    const BlockPosM bp = decode_mfma(g);
+    const int lane = threadIdx.x & 63;
+    const int col = lane & 15, kg = lane >> 4;
+    // Zero-padded tap table: tap_lds[t + 16] = k[t] for t in [0, 2RT].
+    for (int i = threadIdx.x; i < 4 * C + 32; i += 256) {
+        const int t = i - 16;
+        tap_lds[i] = (t >= 0 && t <= 2 * RT) ? taps.k[t] : 0.0f;
+    }
+    __syncthreads();
+    float tv[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) tv[c] = tap_lds[4 * c + kg - col + 16];
+
+    const int64_t n_ax = g.n_ax, in_ax = g.in_ax;
+    const int64_t li = bp.col0 + col;
+    const bool act = li < g.n_lane;
+    const float *src = in + bp.outer * g.in_outer + bp.col0;  // wave-uniform base
+    const int64_t obase = bp.outer * g.out_outer + li * g.out_lane;
+    const int64_t pbase = bp.a0 - RT + kg;  // position of chunk 0 for this lane
+
+    float ring[CR];
+    auto load_chunk = [&](int slot, int64_t G) {
+        const int64_t p = pbase + 4 * G;
+        ring[slot] = (act && p >= 0 && p < n_ax) ? src[p * in_ax + col] : 0.0f;
+    };
+    // Initial fill: the C chunks of tile 0 and the 4 new chunks of tile 1.
+#pragma unroll
+    for (int c = 0; c < C + 4; ++c) load_chunk(c % CR, c);
+
+    const int64_t ntiles = (bp.a1 - bp.a0 + 15) / 16;
+    int64_t t = 0;
+    while (true) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            // tile t = period * P + j; its chunks are 4t + c -> slots (4j + c) % CR
+            v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(tv[c], ring[(4 * j + c) % CR], acc, 0, 0, 0);
+            // prefetch tile t+2's new chunks 4t + C + 4 .. + 7 into freed slots
+#pragma unroll
+            for (int q = 0; q < 4; ++q) load_chunk((4 * j + C + 4 + q) % CR, 4 * t + C + 4 + q);
+            if (act) {
+                const int64_t p0 = bp.a0 + 16 * t + 4 * kg;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (p0 + r < bp.a1) epilogue_store(e, obase + (p0 + r) * g.out_ax, acc[r]);
+            }
+            if (++t == ntiles) return;
+        }
+    }
+}
+
+// ------------------------------------------------------ per-voxel, any radius
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_line_generic(const float *__restrict__ in, LineGeom g,
+                                                      Epilogue e, Taps<kMaxTapsGeneric> taps,
+                                                      int R) {
+    const BlockPos bp = decode_valu(g);
+    if (bp.outer >= g.n_outer) return;
+    const int64_t li = bp.lane0 + (threadIdx.x & 63);
+    if (li >= g.n_lane) return;
+    const float *src = in + bp.outer * g.in_outer + li;
+    const int64_t obase = bp.outer * g.out_outer + li * g.out_lane;
+    for (int64_t p = bp.a0; p < bp.a1; ++p) {
+        const int64_t lo = p < R ? -p : -(int64_t)R;
+        const int64_t hi = (g.n_ax - 1 - p) < R ? (g.n_ax - 1 - p) : (int64_t)R;
+        float sum = 0.0f;
+        for (int64_t r = lo; r <= hi; ++r) {
+            const float v = src[(p + r) * g.in_ax];
+            sum = EXACT ? mac_exact(sum, v, taps.k[R + r]) : mac_fma(sum, v, taps.k[R + r]);
+        }
+        epilogue_store(e, obase + p * g.out_ax, sum);
+    }
+}
+
+// ------------------------------------------------------------ element-wise
+__global__ __launch_bounds__(256) void k_init_state(const uint8_t *__restrict__ m,
+                                                    float *__restrict__ u, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        u[i] = m[i] > 0 ? 1.0f : 0.0f;  // cpu:243-248
+}
+
+__global__ __launch_bounds__(256) void k_quantise(const float *__restrict__ u,
+                                                  uint16_t *__restrict__ q, float *__restrict__ f,
+                                                  int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float v = u[i];
+        if (q != nullptr) q[i] = quantise_u16(v);
+        if (f != nullptr) f[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+static inline int64_t grid_stride_blocks(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return b < 1 ? 1 : (b > 65536 ? 65536 : b);
+}
+
+hipError_t launch_init_state(const uint8_t *mask, float *u, int64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_init_state, dim3((unsigned)grid_stride_blocks(n)), dim3(256), 0, s, mask, u, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_quantise(const float *u, uint16_t *q, float *f, int64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_quantise, dim3((unsigned)grid_stride_blocks(n)), dim3(256), 0, s, u, q, f, n);
+    return hipGetLastError();
+}
+
+LineGeom geom_z(int64_t nz, int64_t ny, int64_t nx) {
+    LineGeom g{};
+    g.n_lane = nx; g.n_outer = ny; g.n_ax = nz;
+    g.in_outer = nx; g.in_ax = ny * nx;
+    g.out_lane = 1; g.out_outer = nx; g.out_ax = ny * nx;
+    return g;
+}
+
+LineGeom geom_y(int64_t nz, int64_t ny, int64_t nx) {
+    LineGeom g{};
+    g.n_lane = nx; g.n_outer = nz; g.n_ax = ny;
+    g.in_outer = ny * nx; g.in_ax = nx;
+    g.out_lane = ny; g.out_outer = nx * ny; g.out_ax = 1;  // (z, x, y) order
+    return g;
+}
+
+LineGeom geom_x(int64_t nz, int64_t ny, int64_t nx) {
+    LineGeom g{};
+    g.n_lane = ny; g.n_outer = nz; g.n_ax = nx;
+    g.in_outer = nx * ny; g.in_ax = ny;                    // reads (z, x, y) order
+    g.out_lane = nx; g.out_outer = ny * nx; g.out_ax = 1;  // writes (z, y, x)
+    return g;
+}
+
+// Split lines into segments (multiples of 16 positions) until the launch holds
+// enough threads to fill the chip: 256 CUs x `waves_per_cu` waves.  Each extra
+// segment re-reads 2R halo positions.
+static void plan_segments(LineGeom &g, int64_t lines_per_thread, int waves_per_cu) {
+    const int64_t threads = ((g.n_lane + 63) / 64) * 64 * g.n_outer / lines_per_thread;
+    const int64_t target = 256LL * waves_per_cu * 64;
+    int64_t nseg = threads > 0 ? (target + threads - 1) / threads : 1;
+    const int64_t max_seg = (g.n_ax + 63) / 64;
+    if (nseg > max_seg) nseg = max_seg;
+    if (nseg < 1) nseg = 1;
+    int64_t seg = (g.n_ax + nseg - 1) / nseg;
+    seg = ((seg + 15) / 16) * 16;
+    if (seg < 16) seg = 16;
+    g.seg = seg;
+    g.nseg = (g.n_ax + seg - 1) / seg;
+    if (g.nseg < 1) g.nseg = 1;
+}
+
+template <int RT>
+static Taps<2 * RT + 1> centred_taps(const float *k, int R) {
+    Taps<2 * RT + 1> t;
+    for (int i = 0; i < 2 * RT + 1; ++i) {
+        const int r = i - RT;
+        t.k[i] = (r >= -R && r <= R) ? k[R + r] : 0.0f;
+    }
+    return t;
+}
+
+template <int RT>
+static hipError_t launch_rt(const float *in, LineGeom g, const Epilogue &e, const float *k, int R,
+                            Arith arith, hipStream_t s) {
+    const Taps<2 * RT + 1> t = centred_taps<RT>(k, R);
+    const dim3 block(256);
+    if (arith == Arith::MFMA) {
+        plan_segments(g, 1, 16);
+        // one block per (segment, 64 lines, outer)
+        const int64_t blocks = g.nseg * ((g.n_lane + 63) / 64) * g.n_outer;
+        if (blocks == 0) return hipSuccess;
+        hipLaunchKernelGGL((k_line_mfma<RT>), dim3((unsigned)blocks), block, 0, s, in, g, e, t);
+        return hipGetLastError();
+    }
+    constexpr int B = RT <= 40 ? 16 : 8;
+    plan_segments(g, 1, 8);
+    const int64_t blocks = g.nseg * ((g.n_lane + 63) / 64) * ((g.n_outer + 3) / 4);
+    if (blocks == 0) return hipSuccess;
+    if (arith == Arith::EXACT)
+        hipLaunchKernelGGL((k_line_valu<RT, B, true>), dim3((unsigned)blocks), block, 0, s, in, g, e, t);
+    else
+        hipLaunchKernelGGL((k_line_valu<RT, B, false>), dim3((unsigned)blocks), block, 0, s, in, g, e, t);
+    return hipGetLastError();
+}
+
+static hipError_t launch_generic(const float *in, LineGeom g, const Epilogue &e, const float *k,
+                                 int R, Arith arith, hipStream_t s) {
+    plan_segments(g, 1, 8);
+    const int64_t blocks = g.nseg * ((g.n_lane + 63) / 64) * ((g.n_outer + 3) / 4);
+    if (blocks == 0) return hipSuccess;
+    Taps<kMaxTapsGeneric> t;
+    for (int i = 0; i < kMaxTapsGeneric; ++i) t.k[i] = i < 2 * R + 1 ? k[i] : 0.0f;
+    const dim3 grid((unsigned)blocks), block(256);
+    if (arith == Arith::EXACT)
+        hipLaunchKernelGGL((k_line_generic<true>), grid, block, 0, s, in, g, e, t, R);
+    else
+        hipLaunchKernelGGL((k_line_generic<false>), grid, block, 0, s, in, g, e, t, R);
+    return hipGetLastError();
+}
+
+hipError_t launch_line_pass_f32(const float *in, const LineGeom &g, const Epilogue &e,
+                                const float *taps, int R, Arith arith, hipStream_t s) {
+#define MXB_RT_CASE(rt) \
+    if (R <= rt) return launch_rt<rt>(in, g, e, taps, R, arith, s);
+    MXB_RT_CASE(2)
+    MXB_RT_CASE(4)
+    MXB_RT_CASE(6)
+    MXB_RT_CASE(8)
+    MXB_RT_CASE(10)
+    MXB_RT_CASE(12)
+    MXB_RT_CASE(14)
+    MXB_RT_CASE(16)
+    MXB_RT_CASE(20)
+    MXB_RT_CASE(24)
+    MXB_RT_CASE(28)
+    MXB_RT_CASE(32)
+    MXB_RT_CASE(36)
+    MXB_RT_CASE(40)
+    MXB_RT_CASE(48)
+    MXB_RT_CASE(56)
+    MXB_RT_CASE(60)
+    MXB_RT_CASE(64)
+#undef MXB_RT_CASE
+    return launch_generic(in, g, e, taps, R, arith, s);
+}
+
+}  // namespace mxb

@@ -1,0 +1,69 @@
+"""Build recipe for the oracle (test infrastructure only).
+
+* ``liboracle.so``: the plain-C restatement in ``diffusion_oracle.c``.
+  Flags follow the reference CPU build (src/Makefile:9-18: -O3, OpenMP, no
+  -march) plus ``-ffp-contract=off`` so that no FMA is ever formed.
+* ``_ref/diffusion<ext-suffix>``: the reference's own CPU module, compiled
+  directly from its sources where they lie
+  (src/pybind/diffusion-pybind.cc, which #includes
+  src/lib/cpp/cpu/diffusion.cc and src/lib/cpp/include/*.hh) with the
+  reference's compile flags (src/Makefile:9,18) and the image's pybind11
+  headers.  Output goes only to ``oracle/_ref/`` (git-ignored).  Skipped when
+  the reference tree is absent (the prebuilt file then travels as is).
+
+Usage: ``python oracle/build.py``.
+"""
+import os
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_SRC = "/root/reference/src"
+
+
+def _run(cmd):
+    subprocess.run(cmd, check=True)
+
+
+def build_oracle() -> str:
+    out = os.path.join(HERE, "liboracle.so")
+    src = os.path.join(HERE, "diffusion_oracle.c")
+    _run(["gcc", "-O3", "-fopenmp", "-ffp-contract=off", "-std=c11", "-fPIC", "-shared",
+          "-Wall", "-Wextra", src, "-o", out + ".tmp", "-lm"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def ref_module_path() -> str:
+    suffix = sysconfig.get_config_var("EXT_SUFFIX")
+    return os.path.join(HERE, "_ref", "diffusion" + suffix)
+
+
+def build_ref() -> str | None:
+    """Compile the reference CPU diffusion module from its own sources."""
+    pyb = os.path.join(REF_SRC, "pybind", "diffusion-pybind.cc")
+    if not os.path.exists(pyb):
+        return None
+    import pybind11  # header-only, part of the image
+
+    out = ref_module_path()
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    inc = [
+        "-isystem", pybind11.get_include(),
+        "-isystem", sysconfig.get_paths()["include"],
+        "-I" + os.path.join(REF_SRC, "lib", "cpp", "include"),
+        "-I" + os.path.join(REF_SRC, "lib", "cpp", "cpu"),
+    ]
+    # src/Makefile:9 (CXXFLAGS), :16 and :18 (cpu platform flags), :31 (-mno-tbm)
+    flags = ["-Wno-unknown-pragmas", "-Wno-comment", "-fopenmp", "-shared", "-fPIC",
+             "-std=c++20", "-O3", "-mno-tbm", "-w"]
+    _run(["g++", *flags, *inc, pyb, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+if __name__ == "__main__":
+    print(build_oracle())
+    print(build_ref())
+    sys.exit(0)

@@ -1,0 +1,188 @@
+"""Parity of the HIP path with the oracle and the golden fixtures (needs a GPU).
+
+Bars (FP32 state):
+* precision "fp32_exact": bit-identical to the oracle (the reference CPU
+  arithmetic: r-ascending sums, product rounded then added).
+* precision "fp32" (MFMA) and "fp32_fma": max |u - u_oracle| <= 1e-5 and
+  uint16 within 1 LSB (src/test/test_diffusion.py:92-98).
+* drop-in `diffusion` (uint16): within 1 LSB of the oracle and of the scipy
+  known answers.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN, load_cases
+from maxibone_amd.helpers import gauss_kernel
+from maxibone_amd.synth import WORKLOADS, kat_mask
+
+pytestmark = pytest.mark.gpu
+F32_TOL = 1e-5
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _u16_close(a, b):
+    return np.abs(a.astype(np.int32) - b.astype(np.int32)).max(initial=0) <= 1
+
+
+def _check_f32(gpu, m, k, reps, prec):
+    u_ref, q_ref = oracle.diffusion_f32(m, k, reps)
+    q, u = gpu.diffusion(m, k, reps, precision=prec, u16=True, f32=True)
+    if prec == "fp32_exact":
+        assert np.array_equal(u.view(np.uint32), u_ref.view(np.uint32)), "not bit-identical"
+        assert np.array_equal(q, q_ref)
+    else:
+        err = np.abs(u - u_ref).max(initial=0)
+        assert err <= F32_TOL, f"max-abs {err}"
+        assert _u16_close(q, q_ref)
+    return u, q
+
+
+@pytest.mark.parametrize("prec", ["fp32_exact", "fp32", "fp32_fma"])
+def test_small_fixtures(gpu, golden_small, golden_correlate, prec):
+    for name, m, k, reps, u_sc, q_sc in golden_small + golden_correlate:
+        _say("case", name, m.shape, len(k), reps, prec)
+        u, q = _check_f32(gpu, m, k, reps, prec)
+        assert _u16_close(q, q_sc)
+
+
+def test_drop_in_module_matches_fixtures(gpu, golden_small):
+    from maxibone_amd.diffusion import diffusion
+    for name, m, k, reps, u_sc, q_sc in golden_small:
+        _say("drop-in", name)
+        out = np.zeros(m.shape, np.uint16)
+        diffusion(m, k, out, reps)
+        assert _u16_close(out, q_sc)
+        _, q_ref = oracle.diffusion_f32(m, k, reps)
+        assert _u16_close(out, q_ref)
+    # bool masks and float64 kernels are cast like the reference's forcecast
+    name, m, k, reps, u_sc, q_sc = golden_small[6]
+    out = np.zeros(m.shape, np.uint16)
+    diffusion(m.astype(bool), k.astype(np.float64), out, repititions=reps, verbose=0)
+    assert _u16_close(out, q_sc)
+
+
+RADII = [1, 2, 3, 5, 6, 9, 12, 13, 17, 21, 27, 31, 36, 40, 44, 52, 60, 63, 64, 65, 90]
+
+
+@pytest.mark.parametrize("R", RADII)
+def test_radius_sweep(gpu, R):
+    rng = np.random.default_rng(R)
+    shape = (int(rng.integers(20, 50)), int(rng.integers(20, 70)), int(rng.integers(60, 200)))
+    m = (rng.random(shape) > 0.995).astype(np.uint8)
+    k = gauss_kernel(R / 4.0).astype(np.float32)
+    assert len(k) == 2 * R + 1
+    for prec in ("fp32_exact", "fp32", "fp32_fma"):
+        _say("R", R, shape, prec)
+        _check_f32(gpu, m, k, 2, prec)
+
+
+SHAPES = [(1, 1, 1), (1, 1, 70), (70, 1, 1), (1, 70, 1), (2, 3, 4), (65, 2, 63), (3, 129, 17),
+          (17, 64, 64), (33, 65, 130), (130, 5, 9)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_edge_shapes(gpu, shape):
+    rng = np.random.default_rng(sum(shape))
+    m = (rng.random(shape) > 0.9).astype(np.uint8)
+    for sigma in (1.0, 3.0, 6.0):
+        k = gauss_kernel(sigma).astype(np.float32)
+        for prec in ("fp32_exact", "fp32"):
+            _say("shape", shape, sigma, prec)
+            _check_f32(gpu, m, k, 3, prec)
+
+
+def test_asymmetric_taps(gpu):
+    rng = np.random.default_rng(11)
+    m = (rng.random((30, 40, 90)) > 0.97).astype(np.uint8)
+    for K in (3, 11, 25, 81):
+        k = rng.random(K).astype(np.float32)
+        k /= k.sum() * 1.01
+        for prec in ("fp32_exact", "fp32", "fp32_fma"):
+            _say("asym", K, prec)
+            _check_f32(gpu, m, k, 3, prec)
+
+
+@pytest.mark.parametrize("fixture", ["kat_128", "c1_128"])
+def test_sampled_fixtures(gpu, fixture):
+    d = np.load(os.path.join(GOLDEN, f"{fixture}.npz"))
+    m = kat_mask().astype(np.uint8) if fixture == "kat_128" else WORKLOADS["c1"].mask()
+    reps = int(d["reps"])
+    _say("fixture", fixture, reps)
+    u_ref, q_ref = oracle.diffusion_f32(m, d["kernel"], reps)
+    from maxibone_amd.diffusion import diffusion
+    out = np.zeros(m.shape, np.uint16)
+    diffusion(m, d["kernel"], out, reps)
+    n = m.shape[0]
+    assert _u16_close(out[n // 2], d["slice_q"])
+    assert _u16_close(out[::5, ::5, ::5], d["sub_q"])
+    assert _u16_close(out, q_ref)
+    for prec in ("fp32", "fp32_exact"):
+        q, u = gpu.diffusion(m, d["kernel"], reps, precision=prec, f32=True)
+        if prec == "fp32_exact":
+            assert np.array_equal(u, u_ref)
+        else:
+            assert np.abs(u - u_ref).max() <= F32_TOL
+
+
+@pytest.mark.parametrize("chunk_z,temporal", [(7, 1), (16, 1), (16, 3), (40, 2), (100, 1), (5, 4)])
+def test_chunked_equals_in_memory(gpu, chunk_z, temporal):
+    rng = np.random.default_rng(chunk_z * 10 + temporal)
+    m = (rng.random((61, 33, 70)) > 0.97).astype(np.uint8)
+    k = gauss_kernel(2.2).astype(np.float32)
+    for prec in ("fp32_exact", "fp32"):
+        _say("chunked", chunk_z, temporal, prec)
+        q0, u0 = gpu.diffusion(m, k, 5, precision=prec, f32=True)
+        q1, u1 = gpu.diffusion_chunked(m, k, 5, chunk_z, temporal, precision=prec, f32=True)
+        assert np.array_equal(u0, u1) and np.array_equal(q0, q1)
+    u_ref = oracle.diffusion_chunked_f32(m, k, 5, chunk_z)
+    assert np.array_equal(gpu.diffusion_chunked(m, k, 5, chunk_z, temporal, "fp32_exact", f32=True)[1], u_ref)
+
+
+def test_on_disk_overload(gpu, tmp_path):
+    from maxibone_amd.diffusion import diffusion
+    rng = np.random.default_rng(21)
+    shape = (45, 20, 36)
+    m = (rng.random(shape) > 0.95).astype(np.uint8)
+    k = gauss_kernel(1.8).astype(np.float32)
+    fin, fout = str(tmp_path / "mask.uint8"), str(tmp_path / "out.uint16")
+    m.tofile(fin)
+    _say("on-disk")
+    diffusion(fin, k, fout, shape, (10, 20, 36), 4)
+    got = np.fromfile(fout, np.uint16).reshape(shape)
+    want = np.zeros(shape, np.uint16)
+    diffusion(m, k, want, 4)
+    assert np.array_equal(got, want)
+    _, q_ref = oracle.diffusion_f32(m, k, 4)
+    assert _u16_close(got, q_ref)
+
+
+def test_batched_equals_single(gpu):
+    rng = np.random.default_rng(31)
+    masks, kernels = [], []
+    for i in range(5):
+        shape = tuple(int(s) for s in rng.integers(8, 60, 3))
+        masks.append((rng.random(shape) > 0.97).astype(np.uint8))
+        kernels.append(gauss_kernel(float(rng.uniform(0.7, 5))).astype(np.float32))
+    _say("batched")
+    qs, fs = gpu.diffusion_batched(masks, kernels, 3, f32=True)
+    for m, k, q, f in zip(masks, kernels, qs, fs):
+        u_ref, q_ref = oracle.diffusion_f32(m, k, 3)
+        assert np.abs(f - u_ref).max() <= F32_TOL and _u16_close(q, q_ref)
+
+
+def test_zero_repetitions_and_determinism(gpu):
+    m = WORKLOADS["c1"].mask()
+    k = gauss_kernel(3.0).astype(np.float32)
+    q0, u0 = gpu.diffusion(m, k, 0, f32=True)
+    assert np.array_equal(q0, (m > 0).astype(np.uint16) * 65535)
+    a = gpu.diffusion(m, k, 4, f32=True)[1]
+    b = gpu.diffusion(m, k, 4, f32=True)[1]
+    assert np.array_equal(a, b)
