@@ -40,7 +40,21 @@ def parse():
     p.add_argument("--reps", type=int, default=None, help="override the workload's repetitions")
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="target CPU-baseline duration (0 disables it)")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="run only a plain elementwise copy of 1 GiB (read) + 1 GiB (write), "
+                        "the reference kernel for scaling HBM byte counters")
     return p.parse_args()
+
+
+def calibrate_copy():
+    """One elementwise kernel that reads 1 GiB and writes 1 GiB (float32)."""
+    import torch
+    n = (1 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "elementwise copy", "read_bytes": 4 * n, "write_bytes": 4 * n}))
 
 
 def algorithmic(workload, precision):
@@ -95,6 +109,9 @@ def load_traffic(workload_name, precision, kernel):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)

@@ -126,9 +126,12 @@ static Arith arith_of(int precision) {
 
 static bool is_lowp(int precision) { return precision == MXB_BF16 || precision == MXB_FP16; }
 
+// Every form needs two FP32-sized state buffers: the 16-bit forms use the
+// first half of each for their repetitions and the whole of both for the
+// final FP32 repetition.
 int64_t state_bytes(int precision, int64_t nz, int64_t ny, int64_t nx) {
-    const int64_t n = nz * ny * nx;
-    return is_lowp(precision) ? n * 2 : n * 4;
+    (void)precision;
+    return nz * ny * nx * 4;
 }
 
 // ---------------------------------------------------------------- driver
@@ -173,12 +176,46 @@ static int run_steps_f32(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t 
     return OK;
 }
 
+// 16-bit form: repetitions 1 .. reps-1 keep BF16/FP16 state (z: s0 -> s1,
+// y: s1 -> s0, x: s0 -> s1 + illumination, swap); the state is then widened
+// to FP32 and the last repetition runs in FP32 (MFMA), so the result carries
+// no rounding of its own repetition.
+static int run_steps_lowp(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t nx,
+                          const float *k, int64_t K, int64_t reps, bool bf, void *s0, void *s1,
+                          uint16_t *out_u16, float *out_f32, hipStream_t s) {
+    const int64_t n = nz * ny * nx;
+    const int R = (int)(K / 2);
+    if (n == 0) return OK;
+    if (reps <= 1)
+        return run_steps_f32(d_mask, nz, ny, nx, k, K, reps, Arith::MFMA, (float *)s0, (float *)s1,
+                             true, out_u16, out_f32, nullptr, s);
+    uint16_t *cur = (uint16_t *)s0, *tmp = (uint16_t *)s1;
+    MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_init_lowp(d_mask, cur, n, bf, s); }));
+    const LineGeom gz = geom_z(nz, ny, nx), gy = geom_y(nz, ny, nx), gx = geom_x(nz, ny, nx);
+    for (int64_t t = 0; t < reps - 1; ++t) {
+        MXB_HIP(timed(MXB_K_ZPASS, s, [&] { return launch_line_pass_lowp(cur, tmp, nullptr, gz, k, R, bf, s); }));
+        MXB_HIP(timed(MXB_K_YPASS, s, [&] { return launch_line_pass_lowp(tmp, cur, nullptr, gy, k, R, bf, s); }));
+        MXB_HIP(timed(MXB_K_XPASS, s, [&] { return launch_line_pass_lowp(cur, tmp, d_mask, gx, k, R, bf, s); }));
+        std::swap(cur, tmp);
+    }
+    // widen into the other buffer, then one FP32 repetition
+    float *f0 = (float *)(cur == (uint16_t *)s0 ? s1 : s0);
+    float *f1 = (float *)(cur == (uint16_t *)s0 ? s0 : s1);
+    MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_lowp_to_f32(cur, f0, n, bf, s); }));
+    return run_steps_f32(d_mask, nz, ny, nx, k, K, 1, Arith::MFMA, f0, f1, false, out_u16, out_f32,
+                         nullptr, s);
+}
+
 static int run_steps(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t nx, const float *k,
                      int64_t K, int64_t reps, int precision, void *s0, void *s1,
                      uint16_t *out_u16, float *out_f32, hipStream_t s) {
     if (is_lowp(precision)) {
-        set_error("BF16/FP16 storage is not available in this build");
-        return EINVAL_;
+        if (K / 2 > kMaxRadiusLowp) {
+            set_error("BF16/FP16 forms support kernel radius <= " + std::to_string(kMaxRadiusLowp));
+            return EINVAL_;
+        }
+        return run_steps_lowp(d_mask, nz, ny, nx, k, K, reps, precision == MXB_BF16, s0, s1,
+                              out_u16, out_f32, s);
     }
     return run_steps_f32(d_mask, nz, ny, nx, k, K, reps, arith_of(precision), (float *)s0,
                          (float *)s1, true, out_u16, out_f32, nullptr, s);

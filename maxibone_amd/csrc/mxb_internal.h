@@ -27,6 +27,7 @@ struct LineGeom {
     int64_t out_lane, out_outer, out_ax;  // output strides (also the mask's)
     int64_t seg;                     // positions per thread segment (multiple of 16)
     int64_t nseg;                    // segments per line
+    int32_t vec_out;                 // 1: out_ax == 1 and every output row is 16-byte aligned
 };
 
 // Epilogue of the last pass of a step: illuminate from the mask, write the
@@ -44,6 +45,15 @@ hipError_t launch_line_pass_f32(const float *in, const LineGeom &g, const Epilog
                                 const float *taps, int R, Arith arith, hipStream_t s);
 hipError_t launch_init_state(const uint8_t *mask, float *u, int64_t n, hipStream_t s);
 hipError_t launch_quantise(const float *u, uint16_t *q, float *f, int64_t n, hipStream_t s);
+
+// Reduced-precision passes (mxb_kernels_lowp.hip): 16-bit state in and out,
+// `bf` selects BF16 (else FP16).  `mask` non-null illuminates.
+hipError_t launch_line_pass_lowp(const uint16_t *in, uint16_t *out, const uint8_t *mask,
+                                 const LineGeom &g, const float *taps, int R, bool bf,
+                                 hipStream_t s);
+hipError_t launch_init_lowp(const uint8_t *mask, uint16_t *u, int64_t n, bool bf, hipStream_t s);
+hipError_t launch_lowp_to_f32(const uint16_t *u, float *f, int64_t n, bool bf, hipStream_t s);
+constexpr int kMaxRadiusLowp = 64;
 
 // Plan the three passes of a step over a (nz, ny, nx) volume.
 // z: A -> A, y: A -> T (T is (z, x, y) order), x: T -> A.

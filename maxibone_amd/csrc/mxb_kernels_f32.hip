@@ -143,12 +143,14 @@ __global__ __launch_bounds__(256) void k_line_valu(const float *__restrict__ in,
 
 // ---------------------------------------------------------------- MFMA form
 // Block = 4 waves = 64 consecutive lines at one outer index; wave w owns lines
-// lane0 + 16w .. +15.  Lane l: column j = l % 16, row group kg = l / 16.
+// col0 = lane0 + 16w .. +15.  Lane l: column j = l % 16, row group kg = l / 16.
 //   A operand (taps):   lane holds T_c[i = l%16][kk = l/16] = k[4c + l/16 - l%16]
 //   B operand (inputs): lane holds X_c[kk = l/16][j = l%16]
 //   D (outputs):        lane holds out_j(base + 4*(l/16) + r), r = 0..3
-// The inputs of chunk G (positions a0 - R + 4G + kk) live in ring slot G % CR;
-// the ring holds one tile of prefetch beyond the C chunks a tile consumes.
+// The inputs of chunk G (positions a0 - R + 4G + kk) live in ring slot G % CR.
+// NT tiles are processed together (independent accumulators, so the MFMA
+// pipeline never waits on one accumulation chain), and the ring holds one
+// group of prefetch beyond the chunks a group consumes.
 struct BlockPosM {
     int64_t outer, col0, a0, a1;
 };
@@ -168,18 +170,28 @@ __device__ __forceinline__ BlockPosM decode_mfma(const LineGeom &g) {
     return p;
 }
 
+// 64 zero floats (device globals are zero-initialised).
+__device__ float g_zero_line[64];
+
 template <int RT>
 struct MfmaShape {
-    static constexpr int C = (16 + 2 * RT + 3) / 4;  // chunks per tile
-    static constexpr int CR = ((C + 3) / 4) * 4 + 4;  // ring slots
-    static constexpr int P = CR / 4;                  // tiles per ring period
+    static constexpr int C = (16 + 2 * RT + 3) / 4;              // chunks per tile
+    static constexpr int NT = 2;                                  // tiles per group
+    static constexpr int GC = 4 * NT;                             // new chunks per group
+    static constexpr int USE = C + 4 * (NT - 1);                  // chunks a group reads
+    static constexpr int CR = ((USE + GC - 1) / GC) * GC + GC;    // ring slots
+    static constexpr int P = CR / GC;                             // groups per ring period
 };
+
+__device__ __forceinline__ float select_mask(uint32_t m, int byte, float v) {
+    return ((m >> (8 * byte)) & 0xffu) != 0 ? 1.0f : v;
+}
 
 template <int RT>
 __global__ __launch_bounds__(256) void k_line_mfma(const float *__restrict__ in, LineGeom g,
                                                    Epilogue e, Taps<2 * RT + 1> taps) {
     using S = MfmaShape<RT>;
-    constexpr int C = S::C, CR = S::CR, P = S::P;
+    constexpr int C = S::C, NT = S::NT, GC = S::GC, USE = S::USE, CR = S::CR, P = S::P;
     __shared__ float tap_lds[4 * C + 32];
     const BlockPosM bp = decode_mfma(g);
     const int lane = threadIdx.x & 63;
@@ -194,42 +206,86 @@ __global__ __launch_bounds__(256) void k_line_mfma(const float *__restrict__ in,
 #pragma unroll
     for (int c = 0; c < C; ++c) tv[c] = tap_lds[4 * c + kg - col + 16];
 
-    const int64_t n_ax = g.n_ax, in_ax = g.in_ax;
+    const int64_t n_ax = g.n_ax;
+    const int64_t in_ax_b = g.in_ax * 4;  // bytes between positions (uniform)
     const int64_t li = bp.col0 + col;
     const bool act = li < g.n_lane;
-    const float *src = in + bp.outer * g.in_outer + bp.col0;  // wave-uniform base
-    const int64_t obase = bp.outer * g.out_outer + li * g.out_lane;
-    const int64_t pbase = bp.a0 - RT + kg;  // position of chunk 0 for this lane
+    // Inactive columns read a valid column; their outputs are never stored and
+    // an MFMA column j only feeds output column j.
+    const int64_t lc = act ? li : g.n_lane - 1;
+    const char *wbase = reinterpret_cast<const char *>(in + bp.outer * g.in_outer);  // uniform
+    const uint32_t lcol_b = (uint32_t)(lc * 4);
+    const uint32_t loff = lcol_b + (uint32_t)(kg * in_ax_b);
+    const int64_t q_first = bp.a0 - RT;  // position of chunk 0, row group 0
 
     float ring[CR];
+    // Positions outside the line read from a zero-filled line instead: the
+    // address is selected before the load, so no value waits on a select and
+    // the prefetched loads stay in flight across the next group's MFMAs.
+    const char *zsrc = reinterpret_cast<const char *>(g_zero_line) + 4 * lane;
     auto load_chunk = [&](int slot, int64_t G) {
-        const int64_t p = pbase + 4 * G;
-        ring[slot] = (act && p >= 0 && p < n_ax) ? src[p * in_ax + col] : 0.0f;
+        const int64_t q0 = q_first + 4 * G;  // uniform
+        const char *cbase = wbase + q0 * in_ax_b;
+        const char *addr = cbase + loff;
+        if (!(q0 >= 0 && q0 + 3 < n_ax)) {
+            const int64_t p = q0 + kg;
+            addr = (p >= 0 && p < n_ax) ? addr : zsrc;
+        }
+        ring[slot] = *reinterpret_cast<const float *>(addr);
     };
-    // Initial fill: the C chunks of tile 0 and the 4 new chunks of tile 1.
 #pragma unroll
-    for (int c = 0; c < C + 4; ++c) load_chunk(c % CR, c);
+    for (int c = 0; c < USE + GC; ++c) load_chunk(c % CR, c);
 
+    // Output addressing: lane base = outer*out_outer + li*out_lane + (a0 + 4kg)*out_ax.
+    const int64_t obase = bp.outer * g.out_outer + lc * g.out_lane + (bp.a0 + 4 * kg) * g.out_ax;
     const int64_t ntiles = (bp.a1 - bp.a0 + 15) / 16;
-    int64_t t = 0;
+    const int64_t ngroups = (ntiles + NT - 1) / NT;
+    int64_t gi = 0;
     while (true) {
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-            // tile t = period * P + j; its chunks are 4t + c -> slots (4j + c) % CR
-            v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            v4f acc[NT];
+#pragma unroll
+            for (int u = 0; u < NT; ++u) acc[u] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int c = 0; c < C; ++c)
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(tv[c], ring[(4 * j + c) % CR], acc, 0, 0, 0);
-            // prefetch tile t+2's new chunks 4t + C + 4 .. + 7 into freed slots
 #pragma unroll
-            for (int q = 0; q < 4; ++q) load_chunk((4 * j + C + 4 + q) % CR, 4 * t + C + 4 + q);
-            if (act) {
-                const int64_t p0 = bp.a0 + 16 * t + 4 * kg;
+                for (int u = 0; u < NT; ++u)
+                    acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(tv[c], ring[(GC * j + 4 * u + c) % CR],
+                                                                  acc[u], 0, 0, 0);
+            // prefetch the new chunks of group gi + 2
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    if (p0 + r < bp.a1) epilogue_store(e, obase + (p0 + r) * g.out_ax, acc[r]);
+            for (int q = 0; q < GC; ++q) load_chunk((GC * j + USE + GC + q) % CR, GC * gi + USE + GC + q);
+#pragma unroll
+            for (int u = 0; u < NT; ++u) {
+                const int64_t t = NT * gi + u;
+                const int64_t tpos = bp.a0 + 16 * t;  // uniform
+                if (tpos >= bp.a1) break;
+                const bool full = tpos + 16 <= bp.a1;
+                if (act && full && g.vec_out) {
+                    const int64_t off = obase + 16 * t;
+                    v4f v = acc[u];
+                    if (e.mask != nullptr) {
+                        const uint32_t m = *reinterpret_cast<const uint32_t *>(e.mask + off);
+                        v.x = select_mask(m, 0, v.x);
+                        v.y = select_mask(m, 1, v.y);
+                        v.z = select_mask(m, 2, v.z);
+                        v.w = select_mask(m, 3, v.w);
+                    }
+                    if (e.out_f32 != nullptr) *reinterpret_cast<v4f *>(e.out_f32 + off) = v;
+                    if (e.out_u16 != nullptr) {
+                        const uint32_t lo = (uint32_t)quantise_u16(v.x) | ((uint32_t)quantise_u16(v.y) << 16);
+                        const uint32_t hi = (uint32_t)quantise_u16(v.z) | ((uint32_t)quantise_u16(v.w) << 16);
+                        *reinterpret_cast<uint2 *>(e.out_u16 + off) = make_uint2(lo, hi);
+                    }
+                } else if (act) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (tpos + 4 * kg + r < bp.a1)
+                            epilogue_store(e, obase + (16 * t + r) * g.out_ax, acc[u][r]);
+                }
             }
-            if (++t == ntiles) return;
+            if (++gi == ngroups) return;
         }
     }
 }
@@ -336,6 +392,21 @@ static void plan_segments(LineGeom &g, int64_t lines_per_thread, int waves_per_c
     if (g.nseg < 1) g.nseg = 1;
 }
 
+// Vector epilogue: 4 consecutive outputs per lane are contiguous and aligned.
+static int32_t vec_ok(const LineGeom &g, const Epilogue &e) {
+    if (g.out_ax != 1 || g.out_lane % 4 != 0 || g.out_outer % 4 != 0) return 0;
+    if (e.out_f32 && (reinterpret_cast<uintptr_t>(e.out_f32) % 16) != 0) return 0;
+    if (e.out_u16 && (reinterpret_cast<uintptr_t>(e.out_u16) % 8) != 0) return 0;
+    if (e.mask && (reinterpret_cast<uintptr_t>(e.mask) % 4) != 0) return 0;
+    return 1;
+}
+
+// The MFMA form addresses a lane's inputs with a 32-bit byte offset from a
+// wave-uniform base: 3 positions plus one line group must fit.
+static bool mfma_offsets_fit(const LineGeom &g) {
+    return (g.in_ax * 3 + g.n_lane) * 4 < (int64_t(1) << 32);
+}
+
 template <int RT>
 static Taps<2 * RT + 1> centred_taps(const float *k, int R) {
     Taps<2 * RT + 1> t;
@@ -351,8 +422,10 @@ static hipError_t launch_rt(const float *in, LineGeom g, const Epilogue &e, cons
                             Arith arith, hipStream_t s) {
     const Taps<2 * RT + 1> t = centred_taps<RT>(k, R);
     const dim3 block(256);
+    if (arith == Arith::MFMA && !mfma_offsets_fit(g)) arith = Arith::FMA;
     if (arith == Arith::MFMA) {
         plan_segments(g, 1, 16);
+        g.vec_out = vec_ok(g, e);
         // one block per (segment, 64 lines, outer)
         const int64_t blocks = g.nseg * ((g.n_lane + 63) / 64) * g.n_outer;
         if (blocks == 0) return hipSuccess;

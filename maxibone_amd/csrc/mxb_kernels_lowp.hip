@@ -1,0 +1,361 @@
+// Reduced-precision (BF16 / FP16 state) line passes of the masked 3D diffusion.
+//
+// Same pass structure and data layouts as the FP32 kernels (lines grouped 64
+// to a block, the y-pass writing (z, x, y) order, the x-pass illuminating),
+// but the state is stored in 16 bits and each pass is a banded-Toeplitz
+// contraction on v_mfma_f32_16x16x32_{bf16,f16} with FP32 accumulation:
+//   D[i][j] += T_c[i][kk] * X_c[kk][j],  i = output (16), j = line (16),
+//   kk = input position within a 32-position chunk c,
+//   T_c[i][kk] = k[32c + kk - i] (zero outside the band).
+// Each tap is split into hi + lo 16-bit parts (two MFMAs per chunk), so the
+// taps carry ~16 significant bits and their sum stays 1 to ~2^-17: the
+// 16-bit rounding happens only where the state is stored.
+//
+// Lane layout of the 16x16x32 operands (lane l, g = l / 16, j = l % 16):
+//   A (taps):   T_c[i = l%16][kk = 8g .. 8g+7]
+//   B (inputs): X_c[kk = 8g .. 8g+7][j = l%16]
+//   D:          out_j(tile base + 4g + r), r = 0..3
+// The inputs a lane needs are 8-position blocks; block b (positions
+// a0 - R + 8b .. +7) is used by lanes with g = b mod 2 ... so lane g keeps,
+// in ring slot m, block 2m + g: tile t, chunk c then reads slot t + 2c,
+// which is the same register in every lane.
+#include <cstring>
+
+#include "mxb_internal.h"
+
+namespace mxb {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+
+template <bool BF>
+struct Lowp;
+
+template <>
+struct Lowp<true> {
+    static __device__ __forceinline__ v4f mfma(v8s a, v8s b, v4f c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a),
+                                                       __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ uint16_t from_f32(float v) {
+        return __builtin_bit_cast(uint16_t, (__bf16)v);  // round to nearest even
+    }
+    static __device__ __forceinline__ float to_f32(uint16_t v) {
+        return __builtin_bit_cast(float, (uint32_t)v << 16);
+    }
+};
+
+template <>
+struct Lowp<false> {
+    static __device__ __forceinline__ v4f mfma(v8s a, v8s b, v4f c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, a),
+                                                      __builtin_bit_cast(v8h, b), c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ uint16_t from_f32(float v) {
+        return __builtin_bit_cast(uint16_t, (_Float16)v);
+    }
+    static __device__ __forceinline__ float to_f32(uint16_t v) {
+        return (float)__builtin_bit_cast(_Float16, v);
+    }
+};
+
+// 16-bit tap bit patterns, hi and lo parts, centred in a capacity of 2RT+1.
+template <int N>
+struct TapsL {
+    uint16_t hi[N];
+    uint16_t lo[N];
+};
+
+__device__ uint16_t g_zero_line16[128];
+
+template <int RT>
+struct LowpShape {
+    static constexpr int C = (16 + 2 * RT + 31) / 32;  // 32-position chunks per tile
+    static constexpr int NT = 2;                        // tiles per group
+    static constexpr int CR = 2 * C + 2;                // ring slots (one group of prefetch)
+    static constexpr int P = CR / NT;                   // groups per ring period
+};
+
+struct BlockPosL {
+    int64_t outer, col0, a0, a1;
+};
+
+__device__ __forceinline__ BlockPosL decode_lowp(const LineGeom &g) {
+    int64_t bid = blockIdx.x;
+    const int64_t seg_id = bid % g.nseg;
+    bid /= g.nseg;
+    const int64_t nlb = (g.n_lane + 63) >> 6;
+    const int64_t lb = bid % nlb;
+    bid /= nlb;
+    BlockPosL p;
+    p.outer = bid;
+    p.col0 = lb * 64 + 16 * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    p.a0 = seg_id * g.seg;
+    p.a1 = p.a0 + g.seg < g.n_ax ? p.a0 + g.seg : g.n_ax;
+    return p;
+}
+
+__device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) {
+    return (uint32_t)lo | ((uint32_t)hi << 16);
+}
+
+template <int RT, bool BF>
+__global__ __launch_bounds__(256) void k_line_lowp(const uint16_t *__restrict__ in,
+                                                   uint16_t *__restrict__ out,
+                                                   const uint8_t *__restrict__ mask, LineGeom g,
+                                                   TapsL<2 * RT + 1> taps) {
+    using S = LowpShape<RT>;
+    using L = Lowp<BF>;
+    constexpr int C = S::C, NT = S::NT, CR = S::CR, P = S::P;
+    constexpr int TW = 32 * C + 32;  // tap table width
+    __shared__ uint16_t thi[TW], tlo[TW];
+    const BlockPosL bp = decode_lowp(g);
+    const int lane = threadIdx.x & 63;
+    const int col = lane & 15, kg = lane >> 4;
+    // Zero-padded tables: t*[x + 16] = k[x] for x in [0, 2RT].
+    for (int i = threadIdx.x; i < TW; i += 256) {
+        const int t = i - 16;
+        const bool in_band = t >= 0 && t <= 2 * RT;
+        thi[i] = in_band ? taps.hi[in_band ? t : 0] : 0;
+        tlo[i] = in_band ? taps.lo[in_band ? t : 0] : 0;
+    }
+    __syncthreads();
+    // A operands: T_c[col][8kg + e] = k[32c + 8kg + e - col]
+    v8s ahi[C], alo[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int x = 32 * c + 8 * kg + e - col + 16;
+            ahi[c][e] = (short)thi[x];
+            alo[c][e] = (short)tlo[x];
+        }
+
+    const int64_t n_ax = g.n_ax;
+    const int64_t in_ax_b = g.in_ax * 2;
+    const int64_t li = bp.col0 + col;
+    const bool act = li < g.n_lane;
+    const int64_t lc = act ? li : g.n_lane - 1;
+    const char *wbase = reinterpret_cast<const char *>(in + bp.outer * g.in_outer);  // uniform
+    const uint32_t lcol_b = (uint32_t)(lc * 2);
+    const uint32_t loff = lcol_b + (uint32_t)(8 * kg * in_ax_b);
+    const char *zsrc = reinterpret_cast<const char *>(g_zero_line16) + 2 * lane;
+    const int64_t q_first = bp.a0 - RT;
+
+    v8s ring[CR];
+    // slot m holds, for lane kg, positions q_first + 16m + 8kg + e, e = 0..7
+    auto load_slot = [&](int slot, int64_t m) {
+        const int64_t qb = q_first + 16 * m;  // uniform
+        const bool inside = qb >= 0 && qb + 31 < n_ax;
+        uint16_t v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const char *addr = wbase + (qb + e) * in_ax_b + loff;
+            if (!inside) {
+                const int64_t p = qb + 8 * kg + e;
+                addr = (p >= 0 && p < n_ax) ? addr : zsrc;
+            }
+            v[e] = *reinterpret_cast<const uint16_t *>(addr);
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) ring[slot][e] = (short)v[e];
+    };
+#pragma unroll
+    for (int m = 0; m < CR; ++m) load_slot(m, m);
+
+    const int64_t obase = bp.outer * g.out_outer + lc * g.out_lane + (bp.a0 + 4 * kg) * g.out_ax;
+    const int64_t ntiles = (bp.a1 - bp.a0 + 15) / 16;
+    const int64_t ngroups = (ntiles + NT - 1) / NT;
+    int64_t gi = 0;
+    while (true) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            v4f acc[NT];
+#pragma unroll
+            for (int u = 0; u < NT; ++u) acc[u] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+#pragma unroll
+                for (int u = 0; u < NT; ++u) {
+                    const v8s x = ring[(NT * j + u + 2 * c) % CR];
+                    acc[u] = L::mfma(ahi[c], x, acc[u]);
+                    acc[u] = L::mfma(alo[c], x, acc[u]);
+                }
+            // prefetch the two new slots of group gi + 2 into the slots of group gi
+#pragma unroll
+            for (int q = 0; q < NT; ++q) load_slot((NT * j + q) % CR, NT * gi + CR + q);
+#pragma unroll
+            for (int u = 0; u < NT; ++u) {
+                const int64_t t = NT * gi + u;
+                const int64_t tpos = bp.a0 + 16 * t;
+                if (tpos >= bp.a1) break;
+                if (!act) continue;
+                const bool full = tpos + 16 <= bp.a1;
+                float v[4] = {acc[u][0], acc[u][1], acc[u][2], acc[u][3]};
+                if (full && g.vec_out) {
+                    const int64_t off = obase + 16 * t;
+                    if (mask != nullptr) {
+                        const uint32_t mm = *reinterpret_cast<const uint32_t *>(mask + off);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            if ((mm >> (8 * r)) & 0xffu) v[r] = 1.0f;
+                    }
+                    *reinterpret_cast<uint2 *>(out + off) =
+                        make_uint2(pack2(L::from_f32(v[0]), L::from_f32(v[1])),
+                                   pack2(L::from_f32(v[2]), L::from_f32(v[3])));
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (tpos + 4 * kg + r < bp.a1) {
+                            const int64_t off = obase + (16 * t + r) * g.out_ax;
+                            float w = v[r];
+                            if (mask != nullptr && mask[off] != 0) w = 1.0f;
+                            out[off] = L::from_f32(w);
+                        }
+                }
+            }
+            if (++gi == ngroups) return;
+        }
+    }
+}
+
+template <bool BF>
+__global__ __launch_bounds__(256) void k_init_lowp(const uint8_t *__restrict__ m,
+                                                   uint16_t *__restrict__ u, int64_t n) {
+    const uint16_t one = Lowp<BF>::from_f32(1.0f);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        u[i] = m[i] > 0 ? one : (uint16_t)0;
+}
+
+template <bool BF>
+__global__ __launch_bounds__(256) void k_lowp_to_f32(const uint16_t *__restrict__ u,
+                                                     float *__restrict__ f, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        f[i] = Lowp<BF>::to_f32(u[i]);
+}
+
+// ---------------------------------------------------------------- host side
+static uint16_t bf16_bits(float v) {  // round to nearest even
+    uint32_t x;
+    std::memcpy(&x, &v, 4);
+    const uint32_t lsb = (x >> 16) & 1u;
+    x += 0x7fffu + lsb;
+    return (uint16_t)(x >> 16);
+}
+
+static float bf16_value(uint16_t b) {
+    const uint32_t x = (uint32_t)b << 16;
+    float v;
+    std::memcpy(&v, &x, 4);
+    return v;
+}
+
+static uint16_t f16_bits(float v) {
+    const _Float16 h = (_Float16)v;
+    uint16_t b;
+    std::memcpy(&b, &h, 2);
+    return b;
+}
+
+static float f16_value(uint16_t b) {
+    _Float16 h;
+    std::memcpy(&h, &b, 2);
+    return (float)h;
+}
+
+template <int RT>
+static TapsL<2 * RT + 1> split_taps(const float *k, int R, bool bf) {
+    TapsL<2 * RT + 1> t;
+    for (int i = 0; i < 2 * RT + 1; ++i) {
+        const int r = i - RT;
+        const float v = (r >= -R && r <= R) ? k[R + r] : 0.0f;
+        const uint16_t hi = bf ? bf16_bits(v) : f16_bits(v);
+        const float rest = v - (bf ? bf16_value(hi) : f16_value(hi));
+        t.hi[i] = hi;
+        t.lo[i] = bf ? bf16_bits(rest) : f16_bits(rest);
+    }
+    return t;
+}
+
+static void plan_segments_lowp(LineGeom &g) {
+    const int64_t threads = ((g.n_lane + 63) / 64) * 16 * g.n_outer;  // one wave per 16 lines
+    const int64_t target = 256LL * 16 * 16;
+    int64_t nseg = threads > 0 ? (target + threads - 1) / threads : 1;
+    const int64_t max_seg = (g.n_ax + 63) / 64;
+    if (nseg > max_seg) nseg = max_seg;
+    if (nseg < 1) nseg = 1;
+    int64_t seg = (g.n_ax + nseg - 1) / nseg;
+    seg = ((seg + 31) / 32) * 32;
+    g.seg = seg;
+    g.nseg = (g.n_ax + seg - 1) / seg;
+    if (g.nseg < 1) g.nseg = 1;
+}
+
+template <int RT>
+static hipError_t launch_lowp_rt(const uint16_t *in, uint16_t *out, const uint8_t *mask,
+                                 LineGeom g, const float *k, int R, bool bf, hipStream_t s) {
+    plan_segments_lowp(g);
+    g.vec_out = (g.out_ax == 1 && g.out_lane % 4 == 0 && g.out_outer % 4 == 0 &&
+                 reinterpret_cast<uintptr_t>(out) % 8 == 0 &&
+                 (mask == nullptr || reinterpret_cast<uintptr_t>(mask) % 4 == 0)) ? 1 : 0;
+    const int64_t blocks = g.nseg * ((g.n_lane + 63) / 64) * g.n_outer;
+    if (blocks == 0) return hipSuccess;
+    const TapsL<2 * RT + 1> t = split_taps<RT>(k, R, bf);
+    if (bf)
+        hipLaunchKernelGGL((k_line_lowp<RT, true>), dim3((unsigned)blocks), dim3(256), 0, s, in, out, mask, g, t);
+    else
+        hipLaunchKernelGGL((k_line_lowp<RT, false>), dim3((unsigned)blocks), dim3(256), 0, s, in, out, mask, g, t);
+    return hipGetLastError();
+}
+
+static bool lowp_offsets_fit(const LineGeom &g) {
+    return (g.in_ax * 2 * 31 + g.n_lane * 2) < (int64_t(1) << 32);
+}
+
+hipError_t launch_line_pass_lowp(const uint16_t *in, uint16_t *out, const uint8_t *mask,
+                                 const LineGeom &g, const float *taps, int R, bool bf,
+                                 hipStream_t s) {
+    if (!lowp_offsets_fit(g)) return hipErrorInvalidValue;
+#define MXB_LOWP_CASE(rt) \
+    if (R <= rt) return launch_lowp_rt<rt>(in, out, mask, g, taps, R, bf, s);
+    MXB_LOWP_CASE(4)
+    MXB_LOWP_CASE(8)
+    MXB_LOWP_CASE(12)
+    MXB_LOWP_CASE(16)
+    MXB_LOWP_CASE(24)
+    MXB_LOWP_CASE(32)
+    MXB_LOWP_CASE(40)
+    MXB_LOWP_CASE(48)
+    MXB_LOWP_CASE(64)
+#undef MXB_LOWP_CASE
+    return hipErrorInvalidValue;
+}
+
+static inline int64_t blocks_for(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return b < 1 ? 1 : (b > 65536 ? 65536 : b);
+}
+
+hipError_t launch_init_lowp(const uint8_t *mask, uint16_t *u, int64_t n, bool bf, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (bf)
+        hipLaunchKernelGGL(k_init_lowp<true>, dim3((unsigned)blocks_for(n)), dim3(256), 0, s, mask, u, n);
+    else
+        hipLaunchKernelGGL(k_init_lowp<false>, dim3((unsigned)blocks_for(n)), dim3(256), 0, s, mask, u, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_lowp_to_f32(const uint16_t *u, float *f, int64_t n, bool bf, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (bf)
+        hipLaunchKernelGGL(k_lowp_to_f32<true>, dim3((unsigned)blocks_for(n)), dim3(256), 0, s, u, f, n);
+    else
+        hipLaunchKernelGGL(k_lowp_to_f32<false>, dim3((unsigned)blocks_for(n)), dim3(256), 0, s, u, f, n);
+    return hipGetLastError();
+}
+
+}  // namespace mxb

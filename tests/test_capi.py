@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
 def test_state_bytes():
     L = capi.lib()
     assert L.mxb_state_bytes(capi.PRECISION["fp32"], 4, 5, 6) == 480
-    assert L.mxb_state_bytes(capi.PRECISION["bf16"], 4, 5, 6) == 240
+    assert L.mxb_state_bytes(capi.PRECISION["bf16"], 4, 5, 6) == 480  # FP32 last repetition
 
 
 def test_drop_in_module_signatures():

@@ -186,3 +186,55 @@ def test_zero_repetitions_and_determinism(gpu):
     a = gpu.diffusion(m, k, 4, f32=True)[1]
     b = gpu.diffusion(m, k, 4, f32=True)[1]
     assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------- precision form
+LOWP_TOL = 2e-3  # max-abs vs the FP64 oracle (BF16/FP16 storage, FP32 last repetition)
+
+
+def _lowp_errors(gpu, m, k32, reps, prec):
+    u64 = oracle.diffusion_f64(m, k32.astype(np.float64), reps)
+    q, u = gpu.diffusion(m, k32, reps, precision=prec, f32=True)
+    d = u.astype(np.float64) - u64
+    return np.abs(d).max(), np.sqrt(np.mean(d * d)), q, u
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("sigma,reps", [(1.0, 6), (3.0, 20), (6.0, 10), (10.0, 8)])
+def test_precision_form_error_vs_fp64(gpu, prec, sigma, reps):
+    m = WORKLOADS["c1"].mask()[:80, :96, :112].copy()
+    k = gauss_kernel(sigma).astype(np.float32)
+    mx, rms, q, u = _lowp_errors(gpu, m, k, reps, prec)
+    _say("lowp", prec, sigma, reps, "max", mx, "rms", rms)
+    assert mx <= LOWP_TOL
+    assert np.all(u[m > 0] == 1.0)
+    assert u.min() >= 0.0 and u.max() <= 1.0 + 1e-6
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_precision_form_edges(gpu, prec):
+    rng = np.random.default_rng(41)
+    for shape in [(1, 1, 1), (5, 3, 70), (37, 2, 19), (33, 65, 130)]:
+        m = (rng.random(shape) > 0.9).astype(np.uint8)
+        for sigma in (0.7, 2.5, 12.0, 16.0):
+            k = gauss_kernel(sigma).astype(np.float32)
+            _say("lowp edge", prec, shape, sigma)
+            mx, rms, _, _ = _lowp_errors(gpu, m, k, 4, prec)
+            assert mx <= LOWP_TOL
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_precision_form_one_repetition_is_fp32(gpu, prec):
+    m = WORKLOADS["c1"].mask()[:40, :50, :60].copy()
+    k = gauss_kernel(2.0).astype(np.float32)
+    a = gpu.diffusion(m, k, 1, precision=prec, f32=True)[1]
+    b = gpu.diffusion(m, k, 1, precision="fp32", f32=True)[1]
+    assert np.array_equal(a, b)
+    q0 = gpu.diffusion(m, k, 0, precision=prec)[0]
+    assert np.array_equal(q0, (m > 0).astype(np.uint16) * 65535)
+
+
+def test_precision_form_rejects_large_radius(gpu):
+    m = np.zeros((4, 4, 4), np.uint8)
+    with pytest.raises(ValueError):
+        gpu.diffusion(m, np.ones(2 * 70 + 1, np.float32) / 141, 3, precision="bf16")
