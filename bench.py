@@ -34,7 +34,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="c2", choices=["c1", "c2", "c4"])
+    p.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "batch"])
     p.add_argument("--precision", default="fp32",
                    choices=["fp32", "bf16", "fp16", "fp32_fma", "fp32_exact"])
     p.add_argument("--reps", type=int, default=None, help="override the workload's repetitions")
@@ -96,6 +96,63 @@ def cpu_baseline(workload, target_s):
                       f"{len(k)} taps, {reps} repetitions, {t:.1f} s"}
 
 
+def bench_batch(args):
+    """64 sub-volumes (dims U[64,384], 5-50 spheres, sigma~U[2,10]), 20 repetitions:
+    the captured persistent graph vs per-volume launches, same GPU, same buffers."""
+    import torch
+    from maxibone_amd import capi
+    from maxibone_amd.synth import batched_volumes
+
+    reps = 20 if args.reps is None else args.reps
+    vols = batched_volumes()
+    dev = torch.device("cuda:0")
+    dm = [torch.from_numpy(v.mask).to(dev) for v in vols]
+    oq = [torch.empty(v.mask.shape, dtype=torch.int16, device=dev) for v in vols]
+    kernels = [v.kernel().astype(np.float32) for v in vols]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan = capi.BatchPlan([t.data_ptr() for t in dm], [v.mask.shape for v in vols], kernels, reps,
+                          outs_u16=[t.data_ptr() for t in oq], precision=args.precision)
+    plan.sync()
+    t_plan = time.perf_counter() - t0
+    info = plan.info()
+    vox_iter = float(sum(v.mask.size for v in vols)) * reps
+
+    def timed(mode):
+        for _ in range(args.warmup):
+            plan.run(mode)
+        plan.sync()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            plan.run(mode)
+        plan.sync()
+        return (time.perf_counter() - t) / args.steps
+
+    t_graph = timed(0) if info["persistent"] else None
+    t_single = timed(1)
+    t_main = t_graph if t_graph is not None else t_single
+    rec = {
+        "metric": "Gvoxel*iter/s (masked 3D diffusion, batched)",
+        "value": round(vox_iter / t_main / 1e9, 3),
+        "unit": "Gvoxel*iter/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_main * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.precision.startswith("fp32") else args.precision,
+        "data": "synthetic",
+        "config": {"workload": "64 sub-volumes, dims U[64,384], 5-50 spheres r~U[3,20], "
+                               "sigma~U[2,10], radius round(4 sigma)",
+                   "voxels": int(sum(v.mask.size for v in vols)), "repetitions": reps,
+                   "precision": args.precision, "output": "uint16"},
+        "persistent_graph_ms": None if t_graph is None else round(t_graph * 1e3, 3),
+        "per_volume_launch_ms": round(t_single * 1e3, 3),
+        "speedup_vs_per_volume": None if t_graph is None else round(t_single / t_graph, 3),
+        "autotune": info, "plan_seconds": round(t_plan, 3),
+    }
+    plan.close()
+    print(json.dumps(rec), flush=True)
+
+
 def load_traffic(workload_name, precision, kernel):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -116,6 +173,9 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        bench_batch(args)
+        return
     import torch
     from maxibone_amd import capi
     from maxibone_amd.synth import WORKLOADS

@@ -105,6 +105,34 @@ typedef struct {
 int mxb_diffusion_batched(const mxb_volume_desc *volumes, int n, int64_t repetitions,
                           int precision, int verbose);
 
+/* Device-resident batch.  A plan pads each volume into a pitched workspace,
+ * builds the per-pass work lists, autotunes the persistent kernel's grid and
+ * segment length on one repetition (cached per batch signature, and in the
+ * file named by MXB_TUNE_FILE), and captures the whole repetition loop in a
+ * HIP graph.  (No reference counterpart.) */
+typedef struct {
+    const uint8_t *dev_mask;  /* device, (nz, ny, nx) C-order */
+    int64_t nz, ny, nx;
+    const float *kernel;      /* host, kernel_size taps       */
+    int64_t kernel_size;
+    uint16_t *dev_out_u16;    /* device, may be NULL           */
+    float *dev_out_f32;       /* device, may be NULL           */
+} mxb_device_volume;
+
+typedef struct mxb_batch_plan mxb_batch_plan;
+
+/* stream: a hipStream_t; NULL makes the plan create its own stream. */
+int mxb_batch_create(const mxb_device_volume *volumes, int n, int64_t repetitions, int precision,
+                     void *stream, mxb_batch_plan **plan);
+/* mode 0: the captured graph of persistent passes (FP32 precision);
+ * mode 1: per-volume launches of the single-volume path (any precision). */
+int mxb_batch_run(mxb_batch_plan *plan, int mode);
+/* Wait for the plan's stream. */
+int mxb_batch_sync(mxb_batch_plan *plan);
+int mxb_batch_info(const mxb_batch_plan *plan, int *persistent, int *grid, int *seg,
+                   double *tuned_ms);
+int mxb_batch_destroy(mxb_batch_plan *plan);
+
 /* ---- device-resident form (inputs already in HBM) -------------------------- */
 
 /* Bytes of one state buffer for `precision` (two are needed). */

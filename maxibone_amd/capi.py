@@ -22,6 +22,11 @@ _vp = ctypes.c_void_p
 _LIB = None
 
 
+class DeviceVolume(ctypes.Structure):
+    _fields_ = [("dev_mask", _vp), ("nz", _i64), ("ny", _i64), ("nx", _i64), ("kernel", _vp),
+                ("kernel_size", _i64), ("dev_out_u16", _vp), ("dev_out_f32", _vp)]
+
+
 class VolumeDesc(ctypes.Structure):
     _fields_ = [("mask", _vp), ("nz", _i64), ("ny", _i64), ("nx", _i64), ("kernel", _vp),
                 ("kernel_size", _i64), ("output_u16", _vp), ("output_f32", _vp)]
@@ -50,6 +55,13 @@ def lib():
             "mxb_kernel_times": [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(_i64)],
             "mxb_device_count": [],
+            "mxb_batch_create": [ctypes.POINTER(DeviceVolume), ctypes.c_int, _i64, ctypes.c_int, _vp,
+                                 ctypes.POINTER(_vp)],
+            "mxb_batch_run": [_vp, ctypes.c_int],
+            "mxb_batch_sync": [_vp],
+            "mxb_batch_info": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)],
+            "mxb_batch_destroy": [_vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -138,3 +150,47 @@ def kernel_times(cls):
     n = _i64()
     check(lib().mxb_kernel_times(cls, ctypes.byref(ms), ctypes.byref(n)))
     return ms.value, n.value
+
+
+class BatchPlan:
+    """A device-resident batch (mxb_batch_create): volumes given as device
+    pointers (e.g. torch tensors' data_ptr()), kernels as numpy arrays."""
+
+    def __init__(self, masks_dev, shapes, kernels, reps, outs_u16=None, outs_f32=None,
+                 precision="fp32", stream=None):
+        n = len(shapes)
+        self._kernels = [np.ascontiguousarray(k, dtype=np.float32) for k in kernels]
+        self._vols = (DeviceVolume * n)()
+        for i in range(n):
+            nz, ny, nx = shapes[i]
+            self._vols[i] = DeviceVolume(masks_dev[i], nz, ny, nx, self._kernels[i].ctypes.data,
+                                         self._kernels[i].shape[0],
+                                         outs_u16[i] if outs_u16 else None,
+                                         outs_f32[i] if outs_f32 else None)
+        self._plan = _vp()
+        check(lib().mxb_batch_create(self._vols, n, int(reps), PRECISION[precision], stream,
+                                     ctypes.byref(self._plan)))
+
+    def run(self, mode=0):
+        check(lib().mxb_batch_run(self._plan, int(mode)))
+
+    def sync(self):
+        check(lib().mxb_batch_sync(self._plan))
+
+    def info(self):
+        p, g, s = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = ctypes.c_double()
+        check(lib().mxb_batch_info(self._plan, ctypes.byref(p), ctypes.byref(g), ctypes.byref(s),
+                                   ctypes.byref(ms)))
+        return {"persistent": bool(p.value), "grid": g.value, "seg": s.value, "tuned_ms": ms.value}
+
+    def close(self):
+        if self._plan:
+            lib().mxb_batch_destroy(self._plan)
+            self._plan = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,674 @@
+// Batched diffusion: many independent sub-volumes of uneven shape and radius.
+//
+// One persistent kernel per pass serves every volume: the work list holds one
+// item per (volume, 64-line group, outer index, segment), sorted by cost
+// (tiles x MFMA chunks) so that a grid-stride walk over it balances the
+// blocks; each item runs the LDS-staged line-pass body compiled for the
+// smallest radius bucket that holds the volume's radius.  Each volume's state
+// is kept in a pitched layout (rows padded to 16 bytes) so the body's
+// 16-byte row loads apply to any shape; the mask is copied into the same
+// pitch, and a final per-volume kernel writes the caller's unpadded outputs.
+// The whole repetition loop is captured once in a HIP graph; replays launch it.
+//
+// The grid size and the segment length are chosen by timing candidates on one
+// repetition of the actual batch (autotune); the choice is cached per batch
+// signature in the process (and in the file named by MXB_TUNE_FILE, if set).
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/maxibone_diffusion.h"
+#include "mxb_lds_body.h"
+
+namespace mxb {
+
+// Radius buckets of the persistent kernel (compiled tap capacities).
+static constexpr int kBucketRT[] = {8, 12, 16, 20, 24, 32, 40, 48, 64};
+static constexpr int kNumBuckets = sizeof(kBucketRT) / sizeof(kBucketRT[0]);
+
+struct BatchPassDesc {
+    const float *in;
+    LineGeom g;
+    Epilogue e;
+    int32_t bucket;
+    int32_t taps_off;  // offset of 2*RT+1 centred taps in the tap table
+};
+
+struct BatchItem {
+    int32_t vol, seg, lb, outer;
+};
+
+template <int RT>
+__device__ __forceinline__ void batch_body(const BatchPassDesc &d, const BatchItem &it,
+                                           const float *__restrict__ taps, float *smem) {
+    lds_fill_taps<RT>(smem, taps + d.taps_off);
+    lds_line_body<RT>(d.in, d.g, d.e, it.outer, it.lb, it.seg, smem);
+}
+
+__global__ __launch_bounds__(256) void k_batched_pass(const BatchPassDesc *__restrict__ descs,
+                                                      const BatchItem *__restrict__ items,
+                                                      int n_items,
+                                                      const float *__restrict__ taps) {
+    extern __shared__ float smem[];
+    for (int i = blockIdx.x; i < n_items; i += gridDim.x) {
+        const BatchItem it = items[i];
+        const BatchPassDesc d = descs[it.vol];
+        switch (d.bucket) {
+            case 0: batch_body<8>(d, it, taps, smem); break;
+            case 1: batch_body<12>(d, it, taps, smem); break;
+            case 2: batch_body<16>(d, it, taps, smem); break;
+            case 3: batch_body<20>(d, it, taps, smem); break;
+            case 4: batch_body<24>(d, it, taps, smem); break;
+            case 5: batch_body<32>(d, it, taps, smem); break;
+            case 6: batch_body<40>(d, it, taps, smem); break;
+            case 7: batch_body<48>(d, it, taps, smem); break;
+            default: batch_body<64>(d, it, taps, smem); break;
+        }
+    }
+}
+
+static int smem_floats(int bucket) {
+    switch (bucket) {
+        case 0: return LdsSmem<8>::TOTAL;
+        case 1: return LdsSmem<12>::TOTAL;
+        case 2: return LdsSmem<16>::TOTAL;
+        case 3: return LdsSmem<20>::TOTAL;
+        case 4: return LdsSmem<24>::TOTAL;
+        case 5: return LdsSmem<32>::TOTAL;
+        case 6: return LdsSmem<40>::TOTAL;
+        case 7: return LdsSmem<48>::TOTAL;
+        default: return LdsSmem<64>::TOTAL;
+    }
+}
+
+static int chunks_of(int bucket) { return (16 + 2 * kBucketRT[bucket] + 3) / 4; }
+
+// Pitched-layout initialisation and final unpadding.
+__global__ __launch_bounds__(256) void k_unpad_out(const float *__restrict__ st, int64_t nz,
+                                                   int64_t ny, int64_t nx, int64_t px,
+                                                   uint16_t *__restrict__ q, float *__restrict__ f) {
+    const int64_t n = nz * ny * nx;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int64_t x = i % nx, zy = i / nx;
+        const float v = st[zy * px + x];
+        if (q != nullptr) q[i] = quantise_u16_l(v);
+        if (f != nullptr) f[i] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_init_pitched(const uint8_t *__restrict__ m, float *__restrict__ u,
+                                                      int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        u[i] = m[i] > 0 ? 1.0f : 0.0f;
+}
+
+static unsigned blocks_for(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
+}
+
+// ----------------------------------------------------------------- the plan
+struct VolLayout {
+    int64_t nz, ny, nx, px, py;
+    int64_t st_off;    // floats, into each state buffer
+    int64_t mask_off;  // bytes, into the pitched masks
+    int R, bucket;
+};
+
+struct TuneChoice {
+    int grid = 0;
+    int seg = 0;  // 0: whole lines
+    double ms = 0;
+};
+
+static std::mutex g_tune_mu;
+static std::map<std::string, TuneChoice> g_tune;
+
+static void load_tune_file() {
+    static bool loaded = false;
+    if (loaded) return;
+    loaded = true;
+    const char *path = std::getenv("MXB_TUNE_FILE");
+    if (!path) return;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return;
+    char key[512];
+    TuneChoice c;
+    while (std::fscanf(f, "%511s %d %d %lf", key, &c.grid, &c.seg, &c.ms) == 4) g_tune[key] = c;
+    std::fclose(f);
+}
+
+static void save_tune_file() {
+    const char *path = std::getenv("MXB_TUNE_FILE");
+    if (!path) return;
+    FILE *f = std::fopen(path, "w");
+    if (!f) return;
+    for (auto &kv : g_tune) std::fprintf(f, "%s %d %d %.6f\n", kv.first.c_str(), kv.second.grid, kv.second.seg, kv.second.ms);
+    std::fclose(f);
+}
+
+}  // namespace mxb
+
+struct mxb_batch_plan {
+    int n = 0;
+    int64_t reps = 0;
+    int precision = MXB_FP32;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::vector<mxb_device_volume> vols;
+    std::vector<mxb::VolLayout> lay;
+    std::vector<std::vector<float>> host_taps;
+    void *ws = nullptr;  // one device workspace
+    uint8_t *masks = nullptr;
+    float *st[2] = {nullptr, nullptr};
+    float *taps = nullptr;
+    mxb::BatchPassDesc *descs = nullptr;  // [6][n]: (parity, pass)
+    mxb::BatchItem *items = nullptr;      // z | y | x lists
+    int n_items[3] = {0, 0, 0};
+    int64_t item_off[3] = {0, 0, 0};
+    int item_cap = 0;
+    int smem_bytes = 0;
+    mxb::TuneChoice tune;
+    bool persistent = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::string key;
+};
+
+namespace mxb {
+
+#define MXB_BHIP(expr)                                                                  \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess) {                                                         \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(_e));               \
+            return _e == hipErrorOutOfMemory ? ENOMEM_ : EDEVICE;                       \
+        }                                                                               \
+    } while (0)
+
+// Geometry of pass `p` (0 z, 1 y, 2 x) of a pitched volume.
+static LineGeom pitched_geom(const VolLayout &v, int p) {
+    LineGeom g{};
+    if (p == 0) {
+        g.n_lane = v.nx; g.n_outer = v.ny; g.n_ax = v.nz;
+        g.in_outer = v.px; g.in_ax = v.ny * v.px;
+        g.out_lane = 1; g.out_outer = v.px; g.out_ax = v.ny * v.px;
+    } else if (p == 1) {
+        g.n_lane = v.nx; g.n_outer = v.nz; g.n_ax = v.ny;
+        g.in_outer = v.ny * v.px; g.in_ax = v.px;
+        g.out_lane = v.py; g.out_outer = v.nx * v.py; g.out_ax = 1;
+    } else {
+        g.n_lane = v.ny; g.n_outer = v.nz; g.n_ax = v.nx;
+        g.in_outer = v.nx * v.py; g.in_ax = v.py;
+        g.out_lane = v.px; g.out_outer = v.ny * v.px; g.out_ax = 1;
+    }
+    return g;
+}
+
+static void set_segments(LineGeom &g, int seg) {
+    const int64_t full = ((g.n_ax + 15) / 16) * 16;
+    g.seg = (seg <= 0 || seg >= full) ? full : seg;
+    g.nseg = (g.n_ax + g.seg - 1) / g.seg;
+}
+
+static void set_out_flags(LineGeom &g, const Epilogue &e) {
+    g.vec_out = (g.out_ax == 1 && g.out_lane % 4 == 0 && g.out_outer % 4 == 0) ? 1 : 0;
+    g.stage_out = (g.out_lane == 1 && g.out_outer % 4 == 0 && g.out_ax % 4 == 0 && e.mask == nullptr &&
+                   e.out_u16 == nullptr && e.out_f32 != nullptr) ? 1 : 0;
+}
+
+// Build descriptors and work lists for segment length `seg` and upload them.
+static int build_tables(mxb_batch_plan *P, int seg) {
+    const int n = P->n;
+    std::vector<BatchPassDesc> d(6 * (size_t)n);
+    std::vector<BatchItem> lists[3];
+    for (int v = 0; v < n; ++v) {
+        const VolLayout &L = P->lay[v];
+        for (int parity = 0; parity < 2; ++parity) {
+            float *cur = P->st[parity], *tmp = P->st[1 - parity];
+            cur += L.st_off;
+            tmp += L.st_off;
+            // z: cur -> tmp, y: tmp -> cur (T order), x: cur -> tmp + illuminate
+            const float *ins[3] = {cur, tmp, cur};
+            float *outs[3] = {tmp, cur, tmp};
+            for (int p = 0; p < 3; ++p) {
+                BatchPassDesc &D = d[(size_t)(parity * 3 + p) * n + v];
+                D.in = ins[p];
+                D.g = pitched_geom(L, p);
+                set_segments(D.g, seg);
+                D.e = Epilogue{p == 2 ? P->masks + L.mask_off : nullptr, outs[p], nullptr};
+                set_out_flags(D.g, D.e);
+                D.bucket = L.bucket;
+                D.taps_off = 0;
+            }
+        }
+    }
+    // taps table: per volume, centred in its bucket capacity
+    std::vector<float> table;
+    std::vector<int32_t> toff(n);
+    for (int v = 0; v < n; ++v) {
+        const int RT = kBucketRT[P->lay[v].bucket], R = P->lay[v].R;
+        toff[v] = (int32_t)table.size();
+        for (int i = 0; i < 2 * RT + 1; ++i) {
+            const int r = i - RT;
+            table.push_back((r >= -R && r <= R) ? P->host_taps[v][R + r] : 0.0f);
+        }
+    }
+    for (auto &D : d) D.taps_off = 0;
+    for (int k = 0; k < 6; ++k)
+        for (int v = 0; v < n; ++v) d[(size_t)k * n + v].taps_off = toff[v];
+    for (int p = 0; p < 3; ++p) {
+        std::vector<std::pair<int64_t, BatchItem>> tmp;
+        for (int v = 0; v < n; ++v) {
+            const LineGeom &g = d[(size_t)p * n + v].g;
+            const int64_t nlb = (g.n_lane + 63) / 64;
+            for (int64_t o = 0; o < g.n_outer; ++o)
+                for (int64_t lb = 0; lb < nlb; ++lb)
+                    for (int64_t s = 0; s < g.nseg; ++s) {
+                        const int64_t a0 = s * g.seg;
+                        const int64_t len = std::min(g.seg, g.n_ax - a0);
+                        const int64_t cost = ((len + 15) / 16 + LdsShape<8>::W) * chunks_of(P->lay[v].bucket);
+                        tmp.push_back({cost, BatchItem{v, (int32_t)s, (int32_t)lb, (int32_t)o}});
+                    }
+        }
+        std::stable_sort(tmp.begin(), tmp.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+        lists[p].reserve(tmp.size());
+        for (auto &t : tmp) lists[p].push_back(t.second);
+    }
+    const int total = (int)(lists[0].size() + lists[1].size() + lists[2].size());
+    if (total > P->item_cap || table.size() > (size_t)(2 * 65 * n + 16)) {
+        set_error("batch work list exceeds its reserved space");
+        return EDEVICE;
+    }
+    int64_t off = 0;
+    for (int p = 0; p < 3; ++p) {
+        P->item_off[p] = off;
+        P->n_items[p] = (int)lists[p].size();
+        MXB_BHIP(hipMemcpyAsync(P->items + off, lists[p].data(), lists[p].size() * sizeof(BatchItem),
+                                hipMemcpyHostToDevice, P->stream));
+        off += (int64_t)lists[p].size();
+    }
+    MXB_BHIP(hipMemcpyAsync(P->descs, d.data(), d.size() * sizeof(BatchPassDesc), hipMemcpyHostToDevice, P->stream));
+    MXB_BHIP(hipMemcpyAsync(P->taps, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice, P->stream));
+    MXB_BHIP(hipStreamSynchronize(P->stream));
+    return OK;
+}
+
+static hipError_t launch_pass(mxb_batch_plan *P, int parity, int p, int grid) {
+    const int n_items = P->n_items[p];
+    if (n_items == 0) return hipSuccess;
+    const unsigned g = (unsigned)std::min(grid, n_items);
+    hipLaunchKernelGGL(k_batched_pass, dim3(g), dim3(256), P->smem_bytes, P->stream,
+                       P->descs + (size_t)(parity * 3 + p) * P->n, P->items + P->item_off[p], n_items,
+                       P->taps);
+    return hipGetLastError();
+}
+
+// Enqueue the whole persistent-form diffusion on P->stream.
+static int enqueue_persistent(mxb_batch_plan *P) {
+    for (int v = 0; v < P->n; ++v) {
+        const VolLayout &L = P->lay[v];
+        const int64_t n = L.nz * L.ny * L.px;
+        hipLaunchKernelGGL(k_init_pitched, dim3(blocks_for(n)), dim3(256), 0, P->stream, P->masks + L.mask_off,
+                           P->st[0] + L.st_off, n);
+        MXB_BHIP(hipGetLastError());
+    }
+    int parity = 0;
+    for (int64_t t = 0; t < P->reps; ++t) {
+        for (int p = 0; p < 3; ++p) MXB_BHIP(launch_pass(P, parity, p, P->tune.grid));
+        parity ^= 1;
+    }
+    for (int v = 0; v < P->n; ++v) {
+        const VolLayout &L = P->lay[v];
+        const mxb_device_volume &V = P->vols[v];
+        const int64_t n = L.nz * L.ny * L.nx;
+        hipLaunchKernelGGL(k_unpad_out, dim3(blocks_for(n)), dim3(256), 0, P->stream, P->st[parity] + L.st_off,
+                           L.nz, L.ny, L.nx, L.px, V.dev_out_u16, V.dev_out_f32);
+        MXB_BHIP(hipGetLastError());
+    }
+    return OK;
+}
+
+// Time one repetition (three passes) with the current tables.
+static int time_one_rep(mxb_batch_plan *P, int grid, double *ms) {
+    hipEvent_t a, b;
+    MXB_BHIP(hipEventCreate(&a));
+    MXB_BHIP(hipEventCreate(&b));
+    for (int p = 0; p < 3; ++p) MXB_BHIP(launch_pass(P, 0, p, grid));  // warm
+    MXB_BHIP(hipEventRecord(a, P->stream));
+    for (int p = 0; p < 3; ++p) MXB_BHIP(launch_pass(P, 0, p, grid));
+    MXB_BHIP(hipEventRecord(b, P->stream));
+    MXB_BHIP(hipEventSynchronize(b));
+    float f = 0;
+    MXB_BHIP(hipEventElapsedTime(&f, a, b));
+    *ms = f;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return OK;
+}
+
+static int autotune(mxb_batch_plan *P, int cus) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    load_tune_file();
+    auto it = g_tune.find(P->key);
+    if (it != g_tune.end()) {
+        P->tune = it->second;
+        return build_tables(P, P->tune.seg);
+    }
+    // init state so that the timed repetitions read finite values
+    for (int v = 0; v < P->n; ++v) {
+        const VolLayout &L = P->lay[v];
+        const int64_t n = L.nz * L.ny * L.px;
+        hipLaunchKernelGGL(k_init_pitched, dim3(blocks_for(n)), dim3(256), 0, P->stream, P->masks + L.mask_off,
+                           P->st[0] + L.st_off, n);
+        MXB_BHIP(hipGetLastError());
+    }
+    TuneChoice best;
+    best.ms = 1e30;
+    const int segs[] = {0, 256, 128};
+    const int per_cu[] = {2, 3, 4, 6, 8};
+    for (int seg : segs) {
+        int rc = build_tables(P, seg);
+        if (rc != OK) return rc;
+        for (int k : per_cu) {
+            const int grid = k * cus;
+            double ms = 0;
+            rc = time_one_rep(P, grid, &ms);
+            if (rc != OK) return rc;
+            if (ms < best.ms) {
+                best.ms = ms;
+                best.grid = grid;
+                best.seg = seg;
+            }
+        }
+    }
+    P->tune = best;
+    g_tune[P->key] = best;
+    save_tune_file();
+    return build_tables(P, best.seg);
+}
+
+static int capture_graph(mxb_batch_plan *P) {
+    MXB_BHIP(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_persistent(P);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(P->stream, &g);
+    if (rc != OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    MXB_BHIP(e);
+    P->graph = g;
+    MXB_BHIP(hipGraphInstantiate(&P->exec, g, nullptr, nullptr, 0));
+    return OK;
+}
+
+static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+}  // namespace mxb
+
+using namespace mxb;
+
+extern "C" {
+
+int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, int precision,
+                     void *stream, mxb_batch_plan **plan_out) {
+    if (plan_out == nullptr || n < 0 || (n > 0 && vols == nullptr) || repetitions < 0) {
+        set_error("mxb_batch_create: invalid arguments");
+        return EINVAL_;
+    }
+    if (precision < MXB_FP32 || precision > MXB_FP32_EXACT) {
+        set_error("unknown precision");
+        return EINVAL_;
+    }
+    *plan_out = nullptr;
+    auto *P = new mxb_batch_plan();
+    P->n = n;
+    P->reps = repetitions;
+    P->precision = precision;
+    P->stream = (hipStream_t)stream;
+    if (P->stream == nullptr) {
+        // graph capture needs a stream of its own
+        if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess) {
+            set_error("mxb_batch_create: stream creation failed");
+            delete P;
+            return EDEVICE;
+        }
+        P->own_stream = true;
+    }
+    P->vols.assign(vols, vols + n);
+    // the persistent form serves FP32 (MFMA) volumes with R <= 64
+    P->persistent = precision == MXB_FP32;
+    int64_t st_total = 0, mask_total = 0;
+    std::string key = "b" + std::to_string(n) + "p" + std::to_string(precision);
+    for (int v = 0; v < n; ++v) {
+        const mxb_device_volume &V = vols[v];
+        if (V.nz < 1 || V.ny < 1 || V.nx < 1 || V.kernel == nullptr || V.kernel_size < 1 ||
+            V.kernel_size % 2 == 0 || V.dev_mask == nullptr || (V.dev_out_u16 == nullptr && V.dev_out_f32 == nullptr)) {
+            set_error("volume " + std::to_string(v) + ": invalid descriptor");
+            mxb_batch_destroy(P);
+            return EINVAL_;
+        }
+        VolLayout L{};
+        L.nz = V.nz; L.ny = V.ny; L.nx = V.nx;
+        L.px = round_up(V.nx, 4);
+        L.py = round_up(V.ny, 4);
+        L.R = (int)(V.kernel_size / 2);
+        if (L.R > 64) P->persistent = false;
+        L.bucket = kNumBuckets - 1;
+        for (int b = 0; b < kNumBuckets; ++b)
+            if (L.R <= kBucketRT[b]) { L.bucket = b; break; }
+        L.st_off = st_total;
+        st_total += round_up(std::max(V.nz * V.ny * L.px, V.nz * V.nx * L.py), 64);
+        L.mask_off = mask_total;
+        mask_total += round_up(V.nz * V.ny * L.px, 256);
+        P->lay.push_back(L);
+        P->host_taps.emplace_back(V.kernel, V.kernel + V.kernel_size);
+        key += "_" + std::to_string(V.nz) + "x" + std::to_string(V.ny) + "x" + std::to_string(V.nx) + "r" +
+               std::to_string(L.R);
+    }
+    P->key = key;
+    if (n == 0) {
+        *plan_out = P;
+        return OK;
+    }
+    // workspace: masks | state0 | state1 | taps | descriptors | items
+    int64_t items_cap = 0;
+    for (auto &L : P->lay) {
+        // an upper bound on items of any pass for any segment choice (seg >= 128)
+        const int64_t m = std::max({L.nz, L.ny, L.nx});
+        items_cap += 3 * ((std::max(L.nx, L.ny) + 63) / 64) * std::max(L.ny, L.nz) * ((m + 127) / 128);
+    }
+    P->item_cap = (int)std::min<int64_t>(items_cap, INT32_MAX / 2);
+    const int64_t taps_cap = 2 * 65 * (int64_t)n + 16;
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    const size_t o_mask = carve((size_t)mask_total);
+    const size_t o_s0 = carve((size_t)st_total * 4);
+    const size_t o_s1 = carve((size_t)st_total * 4);
+    const size_t o_taps = carve((size_t)taps_cap * 4);
+    const size_t o_desc = carve(6 * (size_t)n * sizeof(BatchPassDesc));
+    const size_t o_items = carve((size_t)P->item_cap * sizeof(BatchItem));
+    hipError_t e = hipMalloc(&P->ws, off);
+    if (e != hipSuccess) {
+        set_error(std::string("batch workspace: ") + hipGetErrorString(e));
+        mxb_batch_destroy(P);
+        return e == hipErrorOutOfMemory ? ENOMEM_ : EDEVICE;
+    }
+    char *base = (char *)P->ws;
+    P->masks = (uint8_t *)(base + o_mask);
+    P->st[0] = (float *)(base + o_s0);
+    P->st[1] = (float *)(base + o_s1);
+    P->taps = (float *)(base + o_taps);
+    P->descs = (BatchPassDesc *)(base + o_desc);
+    P->items = (BatchItem *)(base + o_items);
+    int rc = OK;
+    auto fail = [&](int code) {
+        mxb_batch_destroy(P);
+        return code;
+    };
+    // pitched masks (pad columns zero): set-up work, outside any graph
+    if ((e = hipMemsetAsync(P->masks, 0, (size_t)mask_total, P->stream)) != hipSuccess) {
+        set_error(std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+        return fail(EDEVICE);
+    }
+    for (int v = 0; v < n; ++v) {
+        const VolLayout &L = P->lay[v];
+        e = hipMemcpy2DAsync(P->masks + L.mask_off, (size_t)L.px, vols[v].dev_mask, (size_t)L.nx, (size_t)L.nx,
+                             (size_t)(L.nz * L.ny), hipMemcpyDeviceToDevice, P->stream);
+        if (e != hipSuccess) {
+            set_error(std::string("hipMemcpy2DAsync: ") + hipGetErrorString(e));
+            return fail(EDEVICE);
+        }
+    }
+    if (P->persistent) {
+        int smem = 0;
+        for (auto &L : P->lay) smem = std::max(smem, smem_floats(L.bucket) * 4);
+        P->smem_bytes = smem;
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess) {
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+        }
+        if ((rc = autotune(P, cus)) != OK) return fail(rc);
+        if ((rc = capture_graph(P)) != OK) return fail(rc);
+    }
+    *plan_out = P;
+    return OK;
+}
+
+int mxb_batch_run(mxb_batch_plan *P, int mode) {
+    if (P == nullptr) {
+        set_error("null plan");
+        return EINVAL_;
+    }
+    if (P->n == 0) return OK;
+    if (mode == 0 && P->persistent) {
+        MXB_BHIP(hipGraphLaunch(P->exec, P->stream));
+        return OK;
+    }
+    // per-volume launches (also the form used for non-FP32 precisions)
+    for (int v = 0; v < P->n; ++v) {
+        const mxb_device_volume &V = P->vols[v];
+        const VolLayout &L = P->lay[v];
+        int rc = mxb_device_diffusion(V.dev_mask, V.nz, V.ny, V.nx, V.kernel, V.kernel_size, P->reps,
+                                      P->precision, P->st[0] + L.st_off, P->st[1] + L.st_off, V.dev_out_u16,
+                                      V.dev_out_f32, P->stream);
+        if (rc != OK) return rc;
+    }
+    return OK;
+}
+
+int mxb_batch_info(const mxb_batch_plan *P, int *persistent, int *grid, int *seg, double *tuned_ms) {
+    if (P == nullptr) {
+        set_error("null plan");
+        return EINVAL_;
+    }
+    if (persistent) *persistent = P->persistent ? 1 : 0;
+    if (grid) *grid = P->tune.grid;
+    if (seg) *seg = P->tune.seg;
+    if (tuned_ms) *tuned_ms = P->tune.ms;
+    return OK;
+}
+
+int mxb_batch_sync(mxb_batch_plan *P) {
+    if (P == nullptr) {
+        set_error("null plan");
+        return EINVAL_;
+    }
+    MXB_BHIP(hipStreamSynchronize(P->stream));
+    return OK;
+}
+
+int mxb_batch_destroy(mxb_batch_plan *P) {
+    if (P == nullptr) return OK;
+    if (P->stream) (void)hipStreamSynchronize(P->stream);
+    if (P->exec) (void)hipGraphExecDestroy(P->exec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    if (P->ws) (void)hipFree(P->ws);
+    if (P->own_stream) (void)hipStreamDestroy(P->stream);
+    delete P;
+    return OK;
+}
+
+// Host-buffer batch: upload, run the plan (persistent graph for FP32), download.
+int mxb_diffusion_batched(const mxb_volume_desc *volumes, int n, int64_t repetitions, int precision,
+                          int verbose) {
+    (void)verbose;
+    if (n < 0 || (n > 0 && volumes == nullptr)) {
+        set_error("volumes must hold n descriptors");
+        return EINVAL_;
+    }
+    if (n == 0) return OK;
+    std::vector<mxb_device_volume> dv(n);
+    std::vector<void *> bufs;
+    auto cleanup = [&]() {
+        for (void *b : bufs) (void)hipFree(b);
+    };
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const mxb_volume_desc &v = volumes[i];
+        if (v.nz < 1 || v.ny < 1 || v.nx < 1 || v.mask == nullptr ||
+            (v.output_u16 == nullptr && v.output_f32 == nullptr)) {
+            set_error("volume " + std::to_string(i) + ": invalid descriptor");
+            cleanup();
+            return EINVAL_;
+        }
+        const int64_t nv = v.nz * v.ny * v.nx;
+        total += nv;
+        void *m = nullptr, *q = nullptr, *f = nullptr;
+        hipError_t e = hipMalloc(&m, nv);
+        if (e == hipSuccess && v.output_u16) e = hipMalloc(&q, nv * 2);
+        if (e == hipSuccess && v.output_f32) e = hipMalloc(&f, nv * 4);
+        if (m) bufs.push_back(m);
+        if (q) bufs.push_back(q);
+        if (f) bufs.push_back(f);
+        if (e != hipSuccess) {
+            set_error(std::string("batch buffers: ") + hipGetErrorString(e));
+            cleanup();
+            return e == hipErrorOutOfMemory ? ENOMEM_ : EDEVICE;
+        }
+        if ((e = hipMemcpy(m, v.mask, nv, hipMemcpyHostToDevice)) != hipSuccess) {
+            set_error(std::string("hipMemcpy: ") + hipGetErrorString(e));
+            cleanup();
+            return EDEVICE;
+        }
+        dv[i] = mxb_device_volume{(const uint8_t *)m, v.nz, v.ny, v.nx, v.kernel, v.kernel_size,
+                                  (uint16_t *)q, (float *)f};
+    }
+    mxb_batch_plan *P = nullptr;
+    int rc = mxb_batch_create(dv.data(), n, repetitions, precision, nullptr, &P);
+    if (rc == OK) rc = mxb_batch_run(P, 0);
+    if (rc == OK) rc = mxb_batch_sync(P);
+    if (rc == OK) {
+        for (int i = 0; i < n && rc == OK; ++i) {
+            const mxb_volume_desc &v = volumes[i];
+            const int64_t nv = v.nz * v.ny * v.nx;
+            hipError_t e = hipSuccess;
+            if (v.output_u16) e = hipMemcpy(v.output_u16, dv[i].dev_out_u16, nv * 2, hipMemcpyDeviceToHost);
+            if (e == hipSuccess && v.output_f32) e = hipMemcpy(v.output_f32, dv[i].dev_out_f32, nv * 4, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) {
+                set_error(std::string("hipMemcpy: ") + hipGetErrorString(e));
+                rc = EDEVICE;
+            }
+        }
+    }
+    mxb_batch_destroy(P);
+    cleanup();
+    (void)total;
+    return rc;
+}
+
+}  // extern "C"

@@ -608,24 +608,6 @@ int mxb_device_diffusion(const uint8_t *dev_mask, int64_t nz, int64_t ny, int64_
                      dev_state1, dev_out_u16, dev_out_f32, (hipStream_t)stream);
 }
 
-int mxb_diffusion_batched(const mxb_volume_desc *volumes, int n, int64_t repetitions,
-                          int precision, int verbose) {
-    if (n < 0 || (n > 0 && volumes == nullptr)) {
-        set_error("volumes must hold n descriptors");
-        return EINVAL_;
-    }
-    for (int i = 0; i < n; ++i) {
-        const mxb_volume_desc &v = volumes[i];
-        int rc = in_memory(v.mask, v.nz, v.ny, v.nx, v.kernel, v.kernel_size, repetitions,
-                           precision, v.output_u16, v.output_f32, verbose);
-        if (rc != OK) {
-            set_error("volume " + std::to_string(i) + ": " + g_error);
-            return rc;
-        }
-    }
-    return OK;
-}
-
 int mxb_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) {

@@ -28,6 +28,7 @@ struct LineGeom {
     int64_t seg;                     // positions per thread segment (multiple of 16)
     int64_t nseg;                    // segments per line
     int32_t vec_out;                 // 1: out_ax == 1 and every output row is 16-byte aligned
+    int32_t stage_out;               // 1: out_lane == 1, rows 16-byte aligned: stage tiles in LDS
 };
 
 // Epilogue of the last pass of a step: illuminate from the mask, write the
@@ -45,6 +46,12 @@ hipError_t launch_line_pass_f32(const float *in, const LineGeom &g, const Epilog
                                 const float *taps, int R, Arith arith, hipStream_t s);
 hipError_t launch_init_state(const uint8_t *mask, float *u, int64_t n, hipStream_t s);
 hipError_t launch_quantise(const float *u, uint16_t *q, float *f, int64_t n, hipStream_t s);
+
+// LDS-staged FP32 MFMA pass (mxb_kernels_f32_lds.hip): 16-byte loads of whole
+// rows; needs 16-byte-aligned input rows.
+bool lds_pass_supported(const float *in, const LineGeom &g, int R);
+hipError_t launch_line_pass_f32_lds(const float *in, const LineGeom &g, const Epilogue &e,
+                                    const float *taps, int R, hipStream_t s);
 
 // Reduced-precision passes (mxb_kernels_lowp.hip): 16-bit state in and out,
 // `bf` selects BF16 (else FP16).  `mask` non-null illuminates.

@@ -164,18 +164,64 @@ def test_on_disk_overload(gpu, tmp_path):
     assert _u16_close(got, q_ref)
 
 
-def test_batched_equals_single(gpu):
-    rng = np.random.default_rng(31)
+def _batch_inputs(seed, n, lo, hi, s_lo, s_hi):
+    rng = np.random.default_rng(seed)
     masks, kernels = [], []
-    for i in range(5):
-        shape = tuple(int(s) for s in rng.integers(8, 60, 3))
+    for i in range(n):
+        shape = tuple(int(s) for s in rng.integers(lo, hi, 3))
         masks.append((rng.random(shape) > 0.97).astype(np.uint8))
-        kernels.append(gauss_kernel(float(rng.uniform(0.7, 5))).astype(np.float32))
-    _say("batched")
-    qs, fs = gpu.diffusion_batched(masks, kernels, 3, f32=True)
+        kernels.append(gauss_kernel(float(rng.uniform(s_lo, s_hi))).astype(np.float32))
+    return masks, kernels
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_batched_host_entry(gpu, prec):
+    masks, kernels = _batch_inputs(31, 6, 5, 70, 0.7, 10.0)
+    _say("batched host", prec)
+    qs, fs = gpu.diffusion_batched(masks, kernels, 3, precision=prec, f32=True)
     for m, k, q, f in zip(masks, kernels, qs, fs):
-        u_ref, q_ref = oracle.diffusion_f32(m, k, 3)
-        assert np.abs(f - u_ref).max() <= F32_TOL and _u16_close(q, q_ref)
+        if prec == "fp32":
+            u_ref, q_ref = oracle.diffusion_f32(m, k, 3)
+            # the FP32 MFMA chain equals the FMA chain: bit-equal to single-volume runs
+            single = gpu.diffusion(m, k, 3, precision="fp32", f32=True)[1]
+            assert np.array_equal(f, single)
+            assert np.abs(f - u_ref).max() <= F32_TOL and _u16_close(q, q_ref)
+        else:
+            u64 = oracle.diffusion_f64(m, k.astype(np.float64), 3)
+            assert np.abs(f - u64).max() <= LOWP_TOL
+
+
+def test_batched_plan_persistent_graph(gpu):
+    """The captured persistent graph equals per-volume launches, on every replay."""
+    torch = pytest.importorskip("torch")
+    masks, kernels = _batch_inputs(32, 9, 20, 150, 1.0, 10.0)
+    dev = torch.device("cuda:0")
+    dm = [torch.from_numpy(m).to(dev) for m in masks]
+    of = [torch.empty(m.shape, dtype=torch.float32, device=dev) for m in masks]
+    oq = [torch.empty(m.shape, dtype=torch.int16, device=dev) for m in masks]
+    torch.cuda.synchronize()
+    plan = gpu.BatchPlan([t.data_ptr() for t in dm], [m.shape for m in masks], kernels, 4,
+                         outs_u16=[t.data_ptr() for t in oq], outs_f32=[t.data_ptr() for t in of])
+    info = plan.info()
+    _say("batch plan", info)
+    assert info["persistent"] and info["grid"] > 0
+    plan.run(1)
+    plan.sync()
+    want = [t.cpu().numpy().copy() for t in of]
+    wantq = [t.cpu().numpy().copy() for t in oq]
+    for rep in range(3):
+        for t in of:
+            t.fill_(-1.0)
+        torch.cuda.synchronize()
+        plan.run(0)
+        plan.sync()
+        for a, b, qa, qb in zip(of, want, oq, wantq):
+            assert np.array_equal(a.cpu().numpy(), b)
+            assert np.array_equal(qa.cpu().numpy(), qb)
+    for m, k, f in zip(masks, kernels, want):
+        u_ref, _ = oracle.diffusion_f32(m, k, 4)
+        assert np.abs(f - u_ref).max() <= F32_TOL
+    plan.close()
 
 
 def test_zero_repetitions_and_determinism(gpu):
@@ -238,3 +284,17 @@ def test_precision_form_rejects_large_radius(gpu):
     m = np.zeros((4, 4, 4), np.uint8)
     with pytest.raises(ValueError):
         gpu.diffusion(m, np.ones(2 * 70 + 1, np.float32) / 141, 3, precision="bf16")
+
+
+def test_mfma_chain_equals_fma_chain(gpu):
+    """The FP32 MFMA form accumulates taps in order (one chain per output), so
+    it agrees bit for bit with the one-FMA-per-tap VALU form, whatever the
+    tile alignment (in-memory vs chunked)."""
+    rng = np.random.default_rng(77)
+    m = (rng.random((48, 40, 72)) > 0.97).astype(np.uint8)
+    for sigma in (1.5, 4.0, 10.0):
+        k = gauss_kernel(sigma).astype(np.float32)
+        a = gpu.diffusion(m, k, 3, precision="fp32", u16=False, f32=True)[1]
+        b = gpu.diffusion(m, k, 3, precision="fp32_fma", u16=False, f32=True)[1]
+        _say("mfma vs fma", sigma, np.abs(a - b).max())
+        assert np.array_equal(a, b)
