@@ -99,20 +99,18 @@ def cpu_baseline(workload, target_s):
 def bench_batch(args):
     """64 sub-volumes (dims U[64,384], 5-50 spheres, sigma~U[2,10]), 20 repetitions:
     the captured persistent graph vs per-volume launches, same GPU, same buffers."""
-    import torch
-    from maxibone_amd import capi
+    from maxibone_amd import capi, hip
     from maxibone_amd.synth import batched_volumes
 
     reps = 20 if args.reps is None else args.reps
     vols = batched_volumes()
-    dev = torch.device("cuda:0")
-    dm = [torch.from_numpy(v.mask).to(dev) for v in vols]
-    oq = [torch.empty(v.mask.shape, dtype=torch.int16, device=dev) for v in vols]
+    dm = [hip.DeviceBuffer.from_array(v.mask) for v in vols]
+    oq = [hip.DeviceBuffer(2 * v.mask.size) for v in vols]
     kernels = [v.kernel().astype(np.float32) for v in vols]
-    torch.cuda.synchronize()
+    hip.synchronize()
     t0 = time.perf_counter()
-    plan = capi.BatchPlan([t.data_ptr() for t in dm], [v.mask.shape for v in vols], kernels, reps,
-                          outs_u16=[t.data_ptr() for t in oq], precision=args.precision)
+    plan = capi.BatchPlan([b.ptr for b in dm], [v.mask.shape for v in vols], kernels, reps,
+                          outs_u16=[b.ptr for b in oq], precision=args.precision)
     plan.sync()
     t_plan = time.perf_counter() - t0
     info = plan.info()
@@ -158,8 +156,8 @@ def load_traffic(workload_name, precision, kernel):
     if not os.path.exists(path):
         return None
     try:
-        d = json.load(open(path))
-        return d.get(f"{workload_name}/{precision}/{kernel}")
+        d = json.load(open(path)).get(f"{workload_name}/{precision}/{kernel}")
+        return None if d is None else d["bytes_per_launch"]
     except (ValueError, OSError):
         return None
 
@@ -176,42 +174,43 @@ def main():
     if args.workload == "batch":
         bench_batch(args)
         return
-    import torch
-    from maxibone_amd import capi
+    from maxibone_amd import capi, hip
     from maxibone_amd.synth import WORKLOADS
 
     w = WORKLOADS[args.workload]
     reps = w.reps if args.reps is None else args.reps
     L = capi.lib()
-    dev = torch.device("cuda:0")
     mask_h = w.mask()
     nz, ny, nx = mask_h.shape
     n = mask_h.size
     k = w.kernel().astype(np.float32)
     prec = capi.PRECISION[args.precision]
     sbytes = L.mxb_state_bytes(prec, nz, ny, nx)
-    mask_d = torch.from_numpy(mask_h).to(dev)
-    s0 = torch.empty(sbytes, dtype=torch.uint8, device=dev)
-    s1 = torch.empty(sbytes, dtype=torch.uint8, device=dev)
-    out = torch.empty(n, dtype=torch.int16, device=dev)
-    stream = torch.cuda.current_stream(dev)
+    mask_d = hip.DeviceBuffer.from_array(mask_h)
+    s0 = hip.DeviceBuffer(sbytes)
+    s1 = hip.DeviceBuffer(sbytes)
+    out = hip.DeviceBuffer(2 * n)
+    stream = hip.Stream()
 
     def step():
-        capi.check(L.mxb_device_diffusion(mask_d.data_ptr(), nz, ny, nx, k.ctypes.data, len(k), reps,
-                                          prec, s0.data_ptr(), s1.data_ptr(), out.data_ptr(), None,
-                                          stream.cuda_stream))
+        capi.check(L.mxb_device_diffusion(mask_d.ptr, nz, ny, nx, k.ctypes.data, len(k), reps,
+                                          prec, s0.ptr, s1.ptr, out.ptr, None, stream.handle))
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    hip.synchronize()
     capi.set_kernel_timing(True)
-    torch.cuda.synchronize()
+    hip.synchronize()
+    ev0, ev1 = hip.Event(), hip.Event()
     t0 = time.perf_counter()
+    ev0.record(stream)
     for _ in range(args.steps):
         step()
-    torch.cuda.synchronize()
+    ev1.record(stream)
+    hip.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
+    stream_ms = ev0.elapsed_ms(ev1)
     times = {name: capi.kernel_times(c) for name, c in
              (("zpass", capi.K_ZPASS), ("ypass", capi.K_YPASS), ("xpass", capi.K_XPASS),
               ("other", capi.K_OTHER))}
@@ -269,8 +268,9 @@ def main():
                        for kk, vv in per_kernel.items()},
         "roofline": roof,
     }
-    del mask_d, s0, s1, out
-    torch.cuda.empty_cache()
+    rec["stream_ms_per_step"] = round(stream_ms / args.steps, 3)
+    for b in (mask_d, s0, s1, out):
+        b.free()
     if args.cpu_seconds > 0:
         rec["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
     else:

@@ -193,31 +193,29 @@ def test_batched_host_entry(gpu, prec):
 
 def test_batched_plan_persistent_graph(gpu):
     """The captured persistent graph equals per-volume launches, on every replay."""
-    torch = pytest.importorskip("torch")
+    from maxibone_amd import hip
     masks, kernels = _batch_inputs(32, 9, 20, 150, 1.0, 10.0)
-    dev = torch.device("cuda:0")
-    dm = [torch.from_numpy(m).to(dev) for m in masks]
-    of = [torch.empty(m.shape, dtype=torch.float32, device=dev) for m in masks]
-    oq = [torch.empty(m.shape, dtype=torch.int16, device=dev) for m in masks]
-    torch.cuda.synchronize()
-    plan = gpu.BatchPlan([t.data_ptr() for t in dm], [m.shape for m in masks], kernels, 4,
-                         outs_u16=[t.data_ptr() for t in oq], outs_f32=[t.data_ptr() for t in of])
+    dm = [hip.DeviceBuffer.from_array(m) for m in masks]
+    of = [hip.DeviceBuffer(4 * m.size) for m in masks]
+    oq = [hip.DeviceBuffer(2 * m.size) for m in masks]
+    plan = gpu.BatchPlan([b.ptr for b in dm], [m.shape for m in masks], kernels, 4,
+                         outs_u16=[b.ptr for b in oq], outs_f32=[b.ptr for b in of])
     info = plan.info()
     _say("batch plan", info)
     assert info["persistent"] and info["grid"] > 0
     plan.run(1)
     plan.sync()
-    want = [t.cpu().numpy().copy() for t in of]
-    wantq = [t.cpu().numpy().copy() for t in oq]
+    want = [b.to_array(np.float32, m.shape) for b, m in zip(of, masks)]
+    wantq = [b.to_array(np.uint16, m.shape) for b, m in zip(oq, masks)]
     for rep in range(3):
-        for t in of:
-            t.fill_(-1.0)
-        torch.cuda.synchronize()
+        for b in of + oq:
+            b.fill_bytes(0xFF)  # NaN / 65535 until the graph writes
+        hip.synchronize()
         plan.run(0)
         plan.sync()
-        for a, b, qa, qb in zip(of, want, oq, wantq):
-            assert np.array_equal(a.cpu().numpy(), b)
-            assert np.array_equal(qa.cpu().numpy(), qb)
+        for b, w, bq, wq, m in zip(of, want, oq, wantq, masks):
+            assert np.array_equal(b.to_array(np.float32, m.shape), w)
+            assert np.array_equal(bq.to_array(np.uint16, m.shape), wq)
     for m, k, f in zip(masks, kernels, want):
         u_ref, _ = oracle.diffusion_f32(m, k, 4)
         assert np.abs(f - u_ref).max() <= F32_TOL
