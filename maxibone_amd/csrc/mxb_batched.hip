@@ -23,7 +23,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/maxibone_diffusion.h"
+#include "maxibone_diffusion.h"
 #include "mxb_lds_body.h"
 
 namespace mxb {

@@ -15,7 +15,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/maxibone_diffusion.h"
+#include "maxibone_diffusion.h"
 #include "mxb_internal.h"
 
 namespace mxb {

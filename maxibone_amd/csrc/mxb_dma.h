@@ -1,0 +1,54 @@
+// Global -> LDS direct loads (gfx950 global_load_lds_dwordx4) with explicit
+// wait counts.
+//
+// The loads are issued from inline assembly so that the compiler's wait-count
+// tracking does not see them: it would otherwise wait for every outstanding
+// LDS-bound load before any LDS read it cannot prove disjoint, which drains a
+// multi-slot prefetch ring on every iteration.  Callers therefore wait
+// themselves, with vm_wait<N>(): "at most N vector-memory operations of this
+// wave still outstanding".  Vector-memory operations complete in issue order,
+// so N must be at most the number of vector-memory instructions the wave
+// issued after the load being waited for; kernels built on this keep that
+// number a compile-time constant (no conditional loads or stores in the
+// pipelined part of their loops, no register spills).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mxb {
+
+// LDS byte address of a __shared__ object.
+__device__ __forceinline__ uint32_t lds_addr(const void *p) {
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
+}
+
+// 16 bytes per active lane: lane i writes LDS bytes [base + 16 i, +16) from
+// the 16-byte-aligned global address `src`.  `lds_wave_base` must be uniform.
+__device__ __forceinline__ void dma16(uint32_t lds_wave_base, const void *src) {
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+    // one wait state between the SALU write of M0 and the LDS-DMA read of it
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                 :
+                 : "s"(lds_wave_base), "v"(src)
+                 : "memory", "m0");
+#pragma clang diagnostic pop
+}
+
+// s_waitcnt with vmcnt = N and no wait on the other counters.
+template <int N>
+__device__ __forceinline__ void vm_wait() {
+    static_assert(N >= 0 && N < 64, "vmcnt is a 6-bit field");
+    __builtin_amdgcn_s_waitcnt((N & 0xf) | ((N >> 4) << 14) | (0x7 << 4) | (0xf << 8));
+}
+
+// Zero bytes: the source of loads that fall outside the volume, so every
+// lane of every wave issues the same number of loads.  (One copy per
+// translation unit; device globals start zeroed and are never written.)
+static __device__ __attribute__((aligned(16))) uint4 g_dma_zero[4];
+// Destination of the stores of lanes whose line lies beyond the volume (the
+// store is still issued, keeping the per-iteration count fixed).
+static __device__ __attribute__((aligned(16))) uint4 g_dma_sink[64];
+
+}  // namespace mxb

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
+#include <cstring>
 #include <string>
 
 namespace mxb {
@@ -40,6 +42,20 @@ struct Epilogue {
 };
 
 enum class Arith : int { FMA = 0, EXACT = 1, MFMA = 2 };
+
+// MXB_PASS_KERNEL=ring selects the register-ring MFMA kernels even where the
+// LDS-staged ones apply (both compute the same values bit for bit; the switch
+// lets tests compare them).  Read at every launch.
+// MXB_PASS_KERNEL=staged selects, among the LDS-staged kernels, the form that
+// stages loads through VGPRs instead of direct global -> LDS loads.
+inline bool force_ring_kernels() {
+    const char *v = std::getenv("MXB_PASS_KERNEL");
+    return v != nullptr && std::strcmp(v, "ring") == 0;
+}
+inline bool force_vgpr_staging() {
+    const char *v = std::getenv("MXB_PASS_KERNEL");
+    return v != nullptr && std::strcmp(v, "staged") == 0;
+}
 
 // Pass launchers (mxb_kernels_f32.hip).  `taps` are K = 2R+1 host floats.
 hipError_t launch_line_pass_f32(const float *in, const LineGeom &g, const Epilogue &e,

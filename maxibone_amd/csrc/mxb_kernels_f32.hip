@@ -460,7 +460,7 @@ static hipError_t launch_generic(const float *in, LineGeom g, const Epilogue &e,
 
 hipError_t launch_line_pass_f32(const float *in, const LineGeom &g, const Epilogue &e,
                                 const float *taps, int R, Arith arith, hipStream_t s) {
-    if (arith == Arith::MFMA && lds_pass_supported(in, g, R))
+    if (arith == Arith::MFMA && !force_ring_kernels() && lds_pass_supported(in, g, R))
         return launch_line_pass_f32_lds(in, g, e, taps, R, s);
 #define MXB_RT_CASE(rt) \
     if (R <= rt) return launch_rt<rt>(in, g, e, taps, R, arith, s);

@@ -1,78 +1,23 @@
-// Reduced-precision (BF16 / FP16 state) line passes of the masked 3D diffusion.
+// Reduced-precision (BF16 / FP16 state) line passes of the masked 3D diffusion:
+// register-ring form, the fallback for inputs whose rows are not 16-byte
+// aligned (the LDS-staged form is in mxb_kernels_lowp_lds.hip; the arithmetic,
+// chunking and tap split are shared, mxb_lowp.h).
 //
 // Same pass structure and data layouts as the FP32 kernels (lines grouped 64
-// to a block, the y-pass writing (z, x, y) order, the x-pass illuminating),
-// but the state is stored in 16 bits and each pass is a banded-Toeplitz
-// contraction on v_mfma_f32_16x16x32_{bf16,f16} with FP32 accumulation:
-//   D[i][j] += T_c[i][kk] * X_c[kk][j],  i = output (16), j = line (16),
-//   kk = input position within a 32-position chunk c,
-//   T_c[i][kk] = k[32c + kk - i] (zero outside the band).
-// Each tap is split into hi + lo 16-bit parts (two MFMAs per chunk), so the
-// taps carry ~16 significant bits and their sum stays 1 to ~2^-17: the
-// 16-bit rounding happens only where the state is stored.
-//
-// Lane layout of the 16x16x32 operands (lane l, g = l / 16, j = l % 16):
-//   A (taps):   T_c[i = l%16][kk = 8g .. 8g+7]
-//   B (inputs): X_c[kk = 8g .. 8g+7][j = l%16]
-//   D:          out_j(tile base + 4g + r), r = 0..3
+// to a block, the y-pass writing (z, x, y) order, the x-pass illuminating).
 // The inputs a lane needs are 8-position blocks; block b (positions
 // a0 - R + 8b .. +7) is used by lanes with g = b mod 2 ... so lane g keeps,
 // in ring slot m, block 2m + g: tile t, chunk c then reads slot t + 2c,
 // which is the same register in every lane.
-#include <cstring>
-
-#include "mxb_internal.h"
+#include "mxb_lowp.h"
 
 namespace mxb {
-
-typedef float v4f __attribute__((ext_vector_type(4)));
-typedef short v8s __attribute__((ext_vector_type(8)));
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-typedef _Float16 v8h __attribute__((ext_vector_type(8)));
-
-template <bool BF>
-struct Lowp;
-
-template <>
-struct Lowp<true> {
-    static __device__ __forceinline__ v4f mfma(v8s a, v8s b, v4f c) {
-        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a),
-                                                       __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ uint16_t from_f32(float v) {
-        return __builtin_bit_cast(uint16_t, (__bf16)v);  // round to nearest even
-    }
-    static __device__ __forceinline__ float to_f32(uint16_t v) {
-        return __builtin_bit_cast(float, (uint32_t)v << 16);
-    }
-};
-
-template <>
-struct Lowp<false> {
-    static __device__ __forceinline__ v4f mfma(v8s a, v8s b, v4f c) {
-        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, a),
-                                                      __builtin_bit_cast(v8h, b), c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ uint16_t from_f32(float v) {
-        return __builtin_bit_cast(uint16_t, (_Float16)v);
-    }
-    static __device__ __forceinline__ float to_f32(uint16_t v) {
-        return (float)__builtin_bit_cast(_Float16, v);
-    }
-};
-
-// 16-bit tap bit patterns, hi and lo parts, centred in a capacity of 2RT+1.
-template <int N>
-struct TapsL {
-    uint16_t hi[N];
-    uint16_t lo[N];
-};
 
 __device__ uint16_t g_zero_line16[128];
 
 template <int RT>
 struct LowpShape {
-    static constexpr int C = (16 + 2 * RT + 31) / 32;  // 32-position chunks per tile
+    static constexpr int C = LowpChunks<RT>::C;        // 32-position chunks per tile
     static constexpr int NT = 2;                        // tiles per group
     static constexpr int CR = 2 * C + 2;                // ring slots (one group of prefetch)
     static constexpr int P = CR / NT;                   // groups per ring period
@@ -97,10 +42,6 @@ __device__ __forceinline__ BlockPosL decode_lowp(const LineGeom &g) {
     return p;
 }
 
-__device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) {
-    return (uint32_t)lo | ((uint32_t)hi << 16);
-}
-
 template <int RT, bool BF>
 __global__ __launch_bounds__(256) void k_line_lowp(const uint16_t *__restrict__ in,
                                                    uint16_t *__restrict__ out,
@@ -109,29 +50,15 @@ __global__ __launch_bounds__(256) void k_line_lowp(const uint16_t *__restrict__ 
     using S = LowpShape<RT>;
     using L = Lowp<BF>;
     constexpr int C = S::C, NT = S::NT, CR = S::CR, P = S::P;
-    constexpr int TW = 32 * C + 32;  // tap table width
+    constexpr int TW = LowpChunks<RT>::TW;
     __shared__ uint16_t thi[TW], tlo[TW];
     const BlockPosL bp = decode_lowp(g);
     const int lane = threadIdx.x & 63;
     const int col = lane & 15, kg = lane >> 4;
-    // Zero-padded tables: t*[x + 16] = k[x] for x in [0, 2RT].
-    for (int i = threadIdx.x; i < TW; i += 256) {
-        const int t = i - 16;
-        const bool in_band = t >= 0 && t <= 2 * RT;
-        thi[i] = in_band ? taps.hi[in_band ? t : 0] : 0;
-        tlo[i] = in_band ? taps.lo[in_band ? t : 0] : 0;
-    }
+    lowp_fill_taps<RT>(thi, tlo, taps);
     __syncthreads();
-    // A operands: T_c[col][8kg + e] = k[32c + 8kg + e - col]
     v8s ahi[C], alo[C];
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int x = 32 * c + 8 * kg + e - col + 16;
-            ahi[c][e] = (short)thi[x];
-            alo[c][e] = (short)tlo[x];
-        }
+    lowp_read_taps<RT>(thi, tlo, col, kg, ahi, alo);
 
     const int64_t n_ax = g.n_ax;
     const int64_t in_ax_b = g.in_ax * 2;
@@ -239,48 +166,6 @@ __global__ __launch_bounds__(256) void k_lowp_to_f32(const uint16_t *__restrict_
 }
 
 // ---------------------------------------------------------------- host side
-static uint16_t bf16_bits(float v) {  // round to nearest even
-    uint32_t x;
-    std::memcpy(&x, &v, 4);
-    const uint32_t lsb = (x >> 16) & 1u;
-    x += 0x7fffu + lsb;
-    return (uint16_t)(x >> 16);
-}
-
-static float bf16_value(uint16_t b) {
-    const uint32_t x = (uint32_t)b << 16;
-    float v;
-    std::memcpy(&v, &x, 4);
-    return v;
-}
-
-static uint16_t f16_bits(float v) {
-    const _Float16 h = (_Float16)v;
-    uint16_t b;
-    std::memcpy(&b, &h, 2);
-    return b;
-}
-
-static float f16_value(uint16_t b) {
-    _Float16 h;
-    std::memcpy(&h, &b, 2);
-    return (float)h;
-}
-
-template <int RT>
-static TapsL<2 * RT + 1> split_taps(const float *k, int R, bool bf) {
-    TapsL<2 * RT + 1> t;
-    for (int i = 0; i < 2 * RT + 1; ++i) {
-        const int r = i - RT;
-        const float v = (r >= -R && r <= R) ? k[R + r] : 0.0f;
-        const uint16_t hi = bf ? bf16_bits(v) : f16_bits(v);
-        const float rest = v - (bf ? bf16_value(hi) : f16_value(hi));
-        t.hi[i] = hi;
-        t.lo[i] = bf ? bf16_bits(rest) : f16_bits(rest);
-    }
-    return t;
-}
-
 static void plan_segments_lowp(LineGeom &g) {
     const int64_t threads = ((g.n_lane + 63) / 64) * 16 * g.n_outer;  // one wave per 16 lines
     const int64_t target = 256LL * 16 * 16;
@@ -319,6 +204,8 @@ static bool lowp_offsets_fit(const LineGeom &g) {
 hipError_t launch_line_pass_lowp(const uint16_t *in, uint16_t *out, const uint8_t *mask,
                                  const LineGeom &g, const float *taps, int R, bool bf,
                                  hipStream_t s) {
+    if (!force_ring_kernels() && lowp_lds_supported(in, g, R))
+        return launch_line_pass_lowp_lds(in, out, mask, g, taps, R, bf, s);
     if (!lowp_offsets_fit(g)) return hipErrorInvalidValue;
 #define MXB_LOWP_CASE(rt) \
     if (R <= rt) return launch_lowp_rt<rt>(in, out, mask, g, taps, R, bf, s);

@@ -123,11 +123,9 @@ __device__ __forceinline__ void lds_line_body(const float *__restrict__ in, cons
         *reinterpret_cast<v4f *>(&ring[lds_off(si, lr, 4 * lq)]) = v;
     };
 
-    // Fill the first W slots; slots W and W+1 wait in VGPRs (two slots of
-    // global loads stay in flight while tiles compute).
+    // Fill the first W slots, stage slot W in VGPRs.
     for (int s = 0; s < W; ++s) lwrite(s, gload(s));
-    v4f stagedA = gload(W);
-    v4f stagedB = gload(W + 1);
+    v4f staged = gload(W);
     __syncthreads();
 
     float tv[C];
@@ -142,17 +140,10 @@ __device__ __forceinline__ void lds_line_body(const float *__restrict__ in, cons
     const int64_t ntiles = (a1 - a0 + 15) / 16;
 
     for (int64_t t = 0; t < ntiles; ++t) {
-        // 1. slot t + W (loaded two tiles ago) into the ring slot freed by tile
-        //    t-1; 2. global loads for slot t + W + 2 into the freed stage.
-        //    Even tiles use stage A, odd tiles stage B (a uniform branch keeps
-        //    each stage in its own registers).
-        if ((t & 1) == 0) {
-            lwrite(t + W, stagedA);
-            if (t + 2 < ntiles) stagedA = gload(t + W + 2);
-        } else {
-            lwrite(t + W, stagedB);
-            if (t + 2 < ntiles) stagedB = gload(t + W + 2);
-        }
+        // 1. slot t + W (staged last iteration) into the ring slot freed by tile t-1
+        lwrite(t + W, staged);
+        // 2. global loads for slot t + W + 1 (consumed next iteration)
+        if (t + 1 < ntiles) staged = gload(t + W + 1);
         // 3. compute tile t from slots t .. t + W - 1
         int sb[W];
 #pragma unroll

@@ -296,3 +296,46 @@ def test_mfma_chain_equals_fma_chain(gpu):
         b = gpu.diffusion(m, k, 3, precision="fp32_fma", u16=False, f32=True)[1]
         _say("mfma vs fma", sigma, np.abs(a - b).max())
         assert np.array_equal(a, b)
+
+
+@pytest.fixture
+def pass_kernel():
+    """Select the line-pass kernel form through MXB_PASS_KERNEL (read by the
+    library at every launch); restores the default afterwards."""
+    old = os.environ.get("MXB_PASS_KERNEL")
+
+    def use(form):
+        if form:
+            os.environ["MXB_PASS_KERNEL"] = form
+        else:
+            os.environ.pop("MXB_PASS_KERNEL", None)
+
+    yield use
+    use(old)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp32"])
+def test_kernel_forms_agree(gpu, pass_kernel, prec):
+    """The register-ring, VGPR-staged LDS and direct-load LDS forms of a pass
+    group taps into MFMA chunks identically, so they agree bit for bit.
+    Shapes cover segment tails (axes not a multiple of 32), partial 64-line
+    groups, and line strides that route the illuminating pass to another
+    form; the direct-load form runs three times (its pipeline must not race)."""
+    rng = np.random.default_rng(5)
+    shapes = [(64, 64, 128), (37, 40, 208), (33, 24, 72), (20, 8, 48)]
+    for shape in shapes:
+        m = (rng.random(shape) > 0.93).astype(np.uint8)
+        for sigma in (1.2, 5.0, 10.0):
+            k = gauss_kernel(sigma).astype(np.float32)
+            outs = {}
+            for form in ("ring", "staged", ""):
+                pass_kernel(form)
+                outs[form] = gpu.diffusion(m, k, 4, precision=prec, u16=False, f32=True)[1]
+            again = [gpu.diffusion(m, k, 4, precision=prec, u16=False, f32=True)[1] for _ in range(2)]
+            pass_kernel(None)
+            _say("forms", prec, shape, sigma,
+                 {f or "direct": float(np.abs(o - outs["ring"]).max()) for f, o in outs.items()})
+            for f, o in outs.items():
+                assert np.array_equal(o, outs["ring"]), f"form {f or 'direct'} differs"
+            for o in again:
+                assert np.array_equal(o, outs[""])
