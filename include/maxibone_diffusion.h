@@ -135,7 +135,8 @@ int mxb_batch_destroy(mxb_batch_plan *plan);
 
 /* ---- device-resident form (inputs already in HBM) -------------------------- */
 
-/* Bytes of one state buffer for `precision` (two are needed). */
+/* Bytes of one state buffer for `precision` (two are needed); -1 (and
+ * mxb_last_error set) for an unknown precision or a negative extent. */
 int64_t mxb_state_bytes(int precision, int64_t nz, int64_t ny, int64_t nx);
 
 /* Run the diffusion on device buffers, enqueued on `stream` (a hipStream_t;

@@ -477,6 +477,10 @@ const char *mxb_last_error(void) { return g_error.c_str(); }
 const char *mxb_version(void) { return "maxibone_amd diffusion 0.1 (gfx950)"; }
 
 int64_t mxb_state_bytes(int precision, int64_t nz, int64_t ny, int64_t nx) {
+    if (precision < MXB_FP32 || precision > MXB_FP32_EXACT || nz < 0 || ny < 0 || nx < 0) {
+        set_error("mxb_state_bytes: unknown precision or negative extent");
+        return -1;
+    }
     return state_bytes(precision, nz, ny, nx);
 }
 

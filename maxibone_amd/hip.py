@@ -51,6 +51,8 @@ class DeviceBuffer:
 
     def __init__(self, nbytes):
         self.nbytes = int(nbytes)
+        if self.nbytes < 0:
+            raise ValueError(f"negative buffer size {self.nbytes}")
         p = ctypes.c_void_p()
         check(rt().hipMalloc(ctypes.byref(p), max(self.nbytes, 1)))
         self.ptr = p.value

@@ -29,6 +29,8 @@ def test_state_bytes():
     L = capi.lib()
     assert L.mxb_state_bytes(capi.PRECISION["fp32"], 4, 5, 6) == 480
     assert L.mxb_state_bytes(capi.PRECISION["bf16"], 4, 5, 6) == 480  # FP32 last repetition
+    assert L.mxb_state_bytes(1024, 4, 5, 6) == -1   # unknown precision
+    assert L.mxb_state_bytes(capi.PRECISION["fp32"], 4, -5, 6) == -1
 
 
 def test_drop_in_module_signatures():
