@@ -492,7 +492,7 @@ static hipError_t launch_lowp_lds_rt(const uint16_t *in, uint16_t *out, const ui
 }
 
 #ifndef MXB_LOWP_DMA_DEPTH
-#define MXB_LOWP_DMA_DEPTH 4
+#define MXB_LOWP_DMA_DEPTH 5
 #endif
 
 // Direct-load form: needs whole 8-line row pieces (n_lane a multiple of 8) and
