@@ -13,6 +13,7 @@ region.  `cpu_baseline` times the reference's own CPU module (built from its
 sources into oracle/_ref) on a bounded crop of the same workload.
 """
 import argparse
+import dataclasses
 import json
 import os
 import subprocess
@@ -38,6 +39,10 @@ def parse():
     p.add_argument("--precision", default="fp32",
                    choices=["fp32", "bf16", "fp16", "fp32_fma", "fp32_exact"])
     p.add_argument("--reps", type=int, default=None, help="override the workload's repetitions")
+    p.add_argument("--sigma", type=float, default=None,
+                   help="override the workload's sigma (same mask; R = round(4 sigma))")
+    p.add_argument("--no-e2e", action="store_true",
+                   help="skip the end-to-end timing of the drop-in call on host buffers")
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="target CPU-baseline duration (0 disables it)")
     p.add_argument("--calibrate-copy", action="store_true",
@@ -178,6 +183,10 @@ def main():
     from maxibone_amd.synth import WORKLOADS
 
     w = WORKLOADS[args.workload]
+    if args.sigma is not None:
+        w = dataclasses.replace(w, sigma=args.sigma,
+                                description=f"{w.description}; sigma overridden to {args.sigma:g} "
+                                            f"(R={round(4 * args.sigma)})")
     reps = w.reps if args.reps is None else args.reps
     L = capi.lib()
     mask_h = w.mask()
@@ -271,6 +280,18 @@ def main():
     rec["stream_ms_per_step"] = round(stream_ms / args.steps, 3)
     for b in (mask_d, s0, s1, out):
         b.free()
+    if not args.no_e2e:
+        # the drop-in call on host buffers: allocation, H2D of the mask, the
+        # diffusion and D2H of the uint16 result (PCIe-inclusive, not `value`)
+        q = np.empty(mask_h.shape, np.uint16)
+        t0 = time.perf_counter()
+        capi.check(L.mxb_diffusion_ex(mask_h.ctypes.data, nz, ny, nx, k.ctypes.data, len(k), reps,
+                                      prec, q.ctypes.data, None, 0))
+        t_e2e = time.perf_counter() - t0
+        rec["end_to_end"] = {"value": round(float(n) * reps / t_e2e / 1e9, 3), "unit": "Gvoxel*iter/s",
+                             "seconds": round(t_e2e, 3),
+                             "what": "one drop-in call on host buffers incl. device allocation, "
+                                     "H2D of the mask and D2H of the uint16 output"}
     if args.cpu_seconds > 0:
         rec["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
     else:

@@ -18,6 +18,28 @@
 
 namespace mxb {
 
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+typedef float v4f32 __attribute__((ext_vector_type(4)));
+
+// 16-byte global store; NT: non-temporal (streaming) policy.  Measured on
+// MI355X: it helps passes that write whole 256-byte rows and hurts those that
+// write 64-128-byte pieces, which rely on L2 write combining.
+template <bool NT>
+__device__ __forceinline__ void store16(void *p, uint4 v) {
+    const v4u32 x = {v.x, v.y, v.z, v.w};
+    if (NT)
+        __builtin_nontemporal_store(x, reinterpret_cast<v4u32 *>(p));
+    else
+        *reinterpret_cast<v4u32 *>(p) = x;
+}
+template <bool NT>
+__device__ __forceinline__ void store16(float *p, v4f32 v) {
+    if (NT)
+        __builtin_nontemporal_store(v, reinterpret_cast<v4f32 *>(p));
+    else
+        *reinterpret_cast<v4f32 *>(p) = v;
+}
+
 // LDS byte address of a __shared__ object.
 __device__ __forceinline__ uint32_t lds_addr(const void *p) {
     return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
@@ -25,14 +47,22 @@ __device__ __forceinline__ uint32_t lds_addr(const void *p) {
 
 // 16 bytes per active lane: lane i writes LDS bytes [base + 16 i, +16) from
 // the 16-byte-aligned global address `src`.  `lds_wave_base` must be uniform.
+// NT: non-temporal policy (same trade-off as store16).
+template <bool NT>
 __device__ __forceinline__ void dma16(uint32_t lds_wave_base, const void *src) {
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
     // one wait state between the SALU write of M0 and the LDS-DMA read of it
-    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
-                 :
-                 : "s"(lds_wave_base), "v"(src)
-                 : "memory", "m0");
+    if (NT)
+        asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off nt"
+                     :
+                     : "s"(lds_wave_base), "v"(src)
+                     : "memory", "m0");
+    else
+        asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                     :
+                     : "s"(lds_wave_base), "v"(src)
+                     : "memory", "m0");
 #pragma clang diagnostic pop
 }
 

@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void k_line_f32_dma(const float *__restrict__ 
         const bool ok = dl_ok && slot < nslots && p >= 0 && p < n_ax;
         const void *src = ok ? static_cast<const void *>(dsrc + p * in_ax_b)
                              : static_cast<const void *>(g_dma_zero);
-        dma16(ring_lds + (uint32_t)((slot % NSL) * kF32Slot * 4), src);
+        dma16<ZSTAGE>(ring_lds + (uint32_t)((slot % NSL) * kF32Slot * 4), src);
     };
     // ---- mask loads (lanes < 16): line 16 wv + lane, 16 positions
     const bool ml_ok = MASK && line0 + 16 * wv + lane < g.n_lane;
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void k_line_f32_dma(const float *__restrict__ 
         const int64_t tpos = a0 + 16 * (int64_t)tj;
         const bool ok = ml_ok && tpos + 16 <= a1;
         const void *src = ok ? static_cast<const void *>(msrc + tpos) : static_cast<const void *>(g_dma_zero);
-        if (lane < 16) dma16(mring_lds + (uint32_t)((tj % NM) * 1024), src);
+        if (lane < 16) dma16<false>(mring_lds + (uint32_t)((tj % NM) * 1024), src);
     };
     auto issue_group = [&](int i) {
         issue_data(i + W - 1);
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void k_line_f32_dma(const float *__restrict__ 
             const int64_t p = tpos + zr;
             float *op = e.out_f32 + outer * g.out_outer + p * g.out_ax + line0 + 4 * zq;
             if (full) {
-                *reinterpret_cast<v4f *>(zq_ok ? op : sinkf) = v;
+                store16<true>(zq_ok ? op : sinkf, v);
             } else if (zq_ok && p < a1) {
                 *reinterpret_cast<v4f *>(op) = v;
             }
@@ -191,8 +191,8 @@ __global__ __launch_bounds__(256) void k_line_f32_dma(const float *__restrict__ 
                     }
                     if (OUTF) {
                         float *dst = sl_ok ? e.out_f32 + off : sinkf;
-                        *reinterpret_cast<v4f *>(dst) = v[0];
-                        *reinterpret_cast<v4f *>(dst + (sl_ok ? 4 : 0)) = v[1];
+                        store16<false>(dst, v[0]);
+                        store16<false>(dst + (sl_ok ? 4 : 0), v[1]);
                     }
                     if (OUTU) {
                         uint32_t w[4];
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256) void k_line_f32_dma(const float *__restrict__ 
                             w[2 * h] = (uint32_t)quantise_u16_l(v[h][0]) | ((uint32_t)quantise_u16_l(v[h][1]) << 16);
                             w[2 * h + 1] = (uint32_t)quantise_u16_l(v[h][2]) | ((uint32_t)quantise_u16_l(v[h][3]) << 16);
                         }
-                        *reinterpret_cast<uint4 *>(sl_ok ? e.out_u16 + off : sinku) = make_uint4(w[0], w[1], w[2], w[3]);
+                        store16<false>(sl_ok ? e.out_u16 + off : sinku, make_uint4(w[0], w[1], w[2], w[3]));
                     }
                 } else if (sl_ok) {
                     // a segment's last (partial or unpaired) tiles

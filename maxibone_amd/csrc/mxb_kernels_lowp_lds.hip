@@ -325,7 +325,7 @@ __global__ __launch_bounds__(256) void k_line_lowp_dma(const uint16_t *__restric
         const bool ok = dl_ok && slot < nslots && p >= 0 && p < n_ax;
         const void *src = ok ? static_cast<const void *>(dsrc + p * in_ax_b)
                              : static_cast<const void *>(g_dma_zero);
-        dma16(ring_lds + (uint32_t)((slot % NSL) * kSlotBytes), src);
+        dma16<ZSTAGE>(ring_lds + (uint32_t)((slot % NSL) * kSlotBytes), src);
     };
     // ---- mask loads (lanes < 32): line 16 wv + lane / 2, 16 positions each
     const int ml = 16 * wv + (lane >> 1);
@@ -337,7 +337,7 @@ __global__ __launch_bounds__(256) void k_line_lowp_dma(const uint16_t *__restric
         const int64_t ppos = a0 + 32 * (int64_t)pj;
         const bool ok = ml_ok && ppos + 32 <= a1;
         const void *src = ok ? static_cast<const void *>(msrc + ppos) : static_cast<const void *>(g_dma_zero);
-        if (lane < 32) dma16(mring_lds + (uint32_t)((pj % NM) * kMaskSlotBytes), src);
+        if (lane < 32) dma16<false>(mring_lds + (uint32_t)((pj % NM) * kMaskSlotBytes), src);
     };
     auto issue_group = [&](int i) {
         issue_data(i + W - 1);
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(256) void k_line_lowp_dma(const uint16_t *__restric
             const int64_t p = ppos + zr;
             uint16_t *op = out + outer * g.out_outer + p * g.out_ax + line0 + 8 * zq;
             if (full) {
-                *reinterpret_cast<uint4 *>(zq_ok ? op : sink) = v;
+                store16<true>(zq_ok ? op : sink, v);
             } else if (zq_ok && p < a1) {
                 *reinterpret_cast<uint4 *>(op) = v;
             }
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(256) void k_line_lowp_dma(const uint16_t *__restric
                     }
                     v = make_uint4(w[0], w[1], w[2], w[3]);
                 }
-                *reinterpret_cast<uint4 *>(sl_ok ? out + off : sink) = v;
+                store16<false>(sl_ok ? out + off : sink, v);
             } else if (sl_ok && p0 < a1) {
                 const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
