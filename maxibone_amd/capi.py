@@ -154,7 +154,8 @@ def kernel_times(cls):
 
 class BatchPlan:
     """A device-resident batch (mxb_batch_create): volumes given as device
-    pointers (e.g. torch tensors' data_ptr()), kernels as numpy arrays."""
+    pointers of the library's HIP runtime (maxibone_amd.hip buffers), kernels
+    as numpy arrays."""
 
     def __init__(self, masks_dev, shapes, kernels, reps, outs_u16=None, outs_f32=None,
                  precision="fp32", stream=None):
