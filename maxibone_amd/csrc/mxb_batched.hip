@@ -3,11 +3,17 @@
 // One persistent kernel per pass serves every volume: the work list holds one
 // item per (volume, 64-line group, outer index, segment), sorted by cost
 // (tiles x MFMA chunks) so that a grid-stride walk over it balances the
-// blocks; each item runs the LDS-staged line-pass body compiled for the
-// smallest radius bucket that holds the volume's radius.  Each volume's state
-// is kept in a pitched layout (rows padded to 16 bytes) so the body's
-// 16-byte row loads apply to any shape; the mask is copied into the same
-// pitch, and a final per-volume kernel writes the caller's unpadded outputs.
+// blocks; each item runs the direct-load line-pass body (mxb_f32_dma_body.h)
+// compiled for the smallest radius bucket that holds the volume's radius.
+// Each volume's state is kept in a pitched layout, (z, py, px) for the
+// (z, y, x) order and (z, px, py) for the (z, x, y) order, with px, py the
+// extents rounded up to 16, so every pass meets the body's alignment and
+// whole-piece requirements for any shape.  The passes run over the padded
+// line counts (px or py lines); pad lines only ever feed pad outputs (each
+// line is independent and no pass reads positions beyond the true extent),
+// so the valid region does not depend on what the pads hold.  The mask is
+// copied into the same pitch (zero pads), and a final per-volume kernel
+// writes the caller's unpadded outputs.
 // The whole repetition loop is captured once in a HIP graph; replays launch it.
 //
 // The grid size and the segment length are chosen by timing candidates on one
@@ -24,7 +30,7 @@
 #include <vector>
 
 #include "maxibone_diffusion.h"
-#include "mxb_lds_body.h"
+#include "mxb_f32_dma_body.h"
 
 namespace mxb {
 
@@ -44,46 +50,55 @@ struct BatchItem {
     int32_t vol, seg, lb, outer;
 };
 
-template <int RT>
+// Pass forms of the persistent kernel: 0 z (line-contiguous output), 1 y,
+// 2 x (position-contiguous output; x illuminates).
+template <int FORM, int RT>
 __device__ __forceinline__ void batch_body(const BatchPassDesc &d, const BatchItem &it,
-                                           const float *__restrict__ taps, float *smem) {
-    lds_fill_taps<RT>(smem, taps + d.taps_off);
-    lds_line_body<RT>(d.in, d.g, d.e, it.outer, it.lb, it.seg, smem);
+                                           const float *__restrict__ taps, unsigned char *smem) {
+    f32_dma_body<RT, MXB_F32_DMA_DEPTH, FORM == 2, true, false, FORM == 0>(
+        d.in, d.g, d.e, it.outer, it.lb, it.seg, taps + d.taps_off, smem);
 }
 
+template <int FORM>
 __global__ __launch_bounds__(256) void k_batched_pass(const BatchPassDesc *__restrict__ descs,
                                                       const BatchItem *__restrict__ items,
                                                       int n_items,
                                                       const float *__restrict__ taps) {
-    extern __shared__ float smem[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char bsmem[];
     for (int i = blockIdx.x; i < n_items; i += gridDim.x) {
         const BatchItem it = items[i];
         const BatchPassDesc d = descs[it.vol];
         switch (d.bucket) {
-            case 0: batch_body<8>(d, it, taps, smem); break;
-            case 1: batch_body<12>(d, it, taps, smem); break;
-            case 2: batch_body<16>(d, it, taps, smem); break;
-            case 3: batch_body<20>(d, it, taps, smem); break;
-            case 4: batch_body<24>(d, it, taps, smem); break;
-            case 5: batch_body<32>(d, it, taps, smem); break;
-            case 6: batch_body<40>(d, it, taps, smem); break;
-            case 7: batch_body<48>(d, it, taps, smem); break;
-            default: batch_body<64>(d, it, taps, smem); break;
+            case 0: batch_body<FORM, 8>(d, it, taps, bsmem); break;
+            case 1: batch_body<FORM, 12>(d, it, taps, bsmem); break;
+            case 2: batch_body<FORM, 16>(d, it, taps, bsmem); break;
+            case 3: batch_body<FORM, 20>(d, it, taps, bsmem); break;
+            case 4: batch_body<FORM, 24>(d, it, taps, bsmem); break;
+            case 5: batch_body<FORM, 32>(d, it, taps, bsmem); break;
+            case 6: batch_body<FORM, 40>(d, it, taps, bsmem); break;
+            case 7: batch_body<FORM, 48>(d, it, taps, bsmem); break;
+            default: batch_body<FORM, 64>(d, it, taps, bsmem); break;
         }
     }
 }
 
-static int smem_floats(int bucket) {
+template <int FORM, int RT>
+static constexpr int form_smem() {
+    return F32DmaSmem<RT, MXB_F32_DMA_DEPTH, FORM == 2, FORM == 0>::BYTES;
+}
+
+template <int FORM>
+static int smem_bytes_of(int bucket) {
     switch (bucket) {
-        case 0: return LdsSmem<8>::TOTAL;
-        case 1: return LdsSmem<12>::TOTAL;
-        case 2: return LdsSmem<16>::TOTAL;
-        case 3: return LdsSmem<20>::TOTAL;
-        case 4: return LdsSmem<24>::TOTAL;
-        case 5: return LdsSmem<32>::TOTAL;
-        case 6: return LdsSmem<40>::TOTAL;
-        case 7: return LdsSmem<48>::TOTAL;
-        default: return LdsSmem<64>::TOTAL;
+        case 0: return form_smem<FORM, 8>();
+        case 1: return form_smem<FORM, 12>();
+        case 2: return form_smem<FORM, 16>();
+        case 3: return form_smem<FORM, 20>();
+        case 4: return form_smem<FORM, 24>();
+        case 5: return form_smem<FORM, 32>();
+        case 6: return form_smem<FORM, 40>();
+        case 7: return form_smem<FORM, 48>();
+        default: return form_smem<FORM, 64>();
     }
 }
 
@@ -91,15 +106,29 @@ static int chunks_of(int bucket) { return (16 + 2 * kBucketRT[bucket] + 3) / 4; 
 
 // Pitched-layout initialisation and final unpadding.
 __global__ __launch_bounds__(256) void k_unpad_out(const float *__restrict__ st, int64_t nz,
-                                                   int64_t ny, int64_t nx, int64_t px,
+                                                   int64_t ny, int64_t nx, int64_t py, int64_t px,
                                                    uint16_t *__restrict__ q, float *__restrict__ f) {
     const int64_t n = nz * ny * nx;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const int64_t x = i % nx, zy = i / nx;
-        const float v = st[zy * px + x];
+        const int64_t y = zy % ny, z = zy / ny;
+        const float v = st[(z * py + y) * px + x];
         if (q != nullptr) q[i] = quantise_u16_l(v);
         if (f != nullptr) f[i] = v;
+    }
+}
+
+// Copy a (nz, ny, nx) mask into the (nz, py, px) pitch (pads stay as set).
+__global__ __launch_bounds__(256) void k_pitch_mask(const uint8_t *__restrict__ m, int64_t nz,
+                                                    int64_t ny, int64_t nx, int64_t py, int64_t px,
+                                                    uint8_t *__restrict__ dst) {
+    const int64_t n = nz * ny * nx;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int64_t x = i % nx, zy = i / nx;
+        const int64_t y = zy % ny, z = zy / ny;
+        dst[(z * py + y) * px + x] = m[i];
     }
 }
 
@@ -175,7 +204,7 @@ struct mxb_batch_plan {
     int n_items[3] = {0, 0, 0};
     int64_t item_off[3] = {0, 0, 0};
     int item_cap = 0;
-    int smem_bytes = 0;
+    int smem_bytes[3] = {0, 0, 0};  // per pass form
     mxb::TuneChoice tune;
     bool persistent = true;
     hipGraph_t graph = nullptr;
@@ -194,21 +223,23 @@ namespace mxb {
         }                                                                               \
     } while (0)
 
-// Geometry of pass `p` (0 z, 1 y, 2 x) of a pitched volume.
+// Geometry of pass `p` (0 z, 1 y, 2 x) of a pitched volume: (z, py, px)
+// order in and out of z, (z, px, py) between y and x.  Lines run over the
+// padded counts; positions over the true extents; the z-pass skips pad rows.
 static LineGeom pitched_geom(const VolLayout &v, int p) {
     LineGeom g{};
     if (p == 0) {
-        g.n_lane = v.nx; g.n_outer = v.ny; g.n_ax = v.nz;
-        g.in_outer = v.px; g.in_ax = v.ny * v.px;
-        g.out_lane = 1; g.out_outer = v.px; g.out_ax = v.ny * v.px;
+        g.n_lane = v.px; g.n_outer = v.ny; g.n_ax = v.nz;
+        g.in_outer = v.px; g.in_ax = v.py * v.px;
+        g.out_lane = 1; g.out_outer = v.px; g.out_ax = v.py * v.px;
     } else if (p == 1) {
-        g.n_lane = v.nx; g.n_outer = v.nz; g.n_ax = v.ny;
-        g.in_outer = v.ny * v.px; g.in_ax = v.px;
-        g.out_lane = v.py; g.out_outer = v.nx * v.py; g.out_ax = 1;
+        g.n_lane = v.px; g.n_outer = v.nz; g.n_ax = v.ny;
+        g.in_outer = v.py * v.px; g.in_ax = v.px;
+        g.out_lane = v.py; g.out_outer = v.px * v.py; g.out_ax = 1;
     } else {
-        g.n_lane = v.ny; g.n_outer = v.nz; g.n_ax = v.nx;
-        g.in_outer = v.nx * v.py; g.in_ax = v.py;
-        g.out_lane = v.px; g.out_outer = v.ny * v.px; g.out_ax = 1;
+        g.n_lane = v.py; g.n_outer = v.nz; g.n_ax = v.nx;
+        g.in_outer = v.px * v.py; g.in_ax = v.py;
+        g.out_lane = v.px; g.out_outer = v.py * v.px; g.out_ax = 1;
     }
     return g;
 }
@@ -217,12 +248,6 @@ static void set_segments(LineGeom &g, int seg) {
     const int64_t full = ((g.n_ax + 15) / 16) * 16;
     g.seg = (seg <= 0 || seg >= full) ? full : seg;
     g.nseg = (g.n_ax + g.seg - 1) / g.seg;
-}
-
-static void set_out_flags(LineGeom &g, const Epilogue &e) {
-    g.vec_out = (g.out_ax == 1 && g.out_lane % 4 == 0 && g.out_outer % 4 == 0) ? 1 : 0;
-    g.stage_out = (g.out_lane == 1 && g.out_outer % 4 == 0 && g.out_ax % 4 == 0 && e.mask == nullptr &&
-                   e.out_u16 == nullptr && e.out_f32 != nullptr) ? 1 : 0;
 }
 
 // Build descriptors and work lists for segment length `seg` and upload them.
@@ -245,7 +270,6 @@ static int build_tables(mxb_batch_plan *P, int seg) {
                 D.g = pitched_geom(L, p);
                 set_segments(D.g, seg);
                 D.e = Epilogue{p == 2 ? P->masks + L.mask_off : nullptr, outs[p], nullptr};
-                set_out_flags(D.g, D.e);
                 D.bucket = L.bucket;
                 D.taps_off = 0;
             }
@@ -306,9 +330,14 @@ static hipError_t launch_pass(mxb_batch_plan *P, int parity, int p, int grid) {
     const int n_items = P->n_items[p];
     if (n_items == 0) return hipSuccess;
     const unsigned g = (unsigned)std::min(grid, n_items);
-    hipLaunchKernelGGL(k_batched_pass, dim3(g), dim3(256), P->smem_bytes, P->stream,
-                       P->descs + (size_t)(parity * 3 + p) * P->n, P->items + P->item_off[p], n_items,
-                       P->taps);
+    const BatchPassDesc *d = P->descs + (size_t)(parity * 3 + p) * P->n;
+    const BatchItem *it = P->items + P->item_off[p];
+    if (p == 0)
+        hipLaunchKernelGGL(k_batched_pass<0>, dim3(g), dim3(256), P->smem_bytes[0], P->stream, d, it, n_items, P->taps);
+    else if (p == 1)
+        hipLaunchKernelGGL(k_batched_pass<1>, dim3(g), dim3(256), P->smem_bytes[1], P->stream, d, it, n_items, P->taps);
+    else
+        hipLaunchKernelGGL(k_batched_pass<2>, dim3(g), dim3(256), P->smem_bytes[2], P->stream, d, it, n_items, P->taps);
     return hipGetLastError();
 }
 
@@ -316,7 +345,7 @@ static hipError_t launch_pass(mxb_batch_plan *P, int parity, int p, int grid) {
 static int enqueue_persistent(mxb_batch_plan *P) {
     for (int v = 0; v < P->n; ++v) {
         const VolLayout &L = P->lay[v];
-        const int64_t n = L.nz * L.ny * L.px;
+        const int64_t n = L.nz * L.py * L.px;
         hipLaunchKernelGGL(k_init_pitched, dim3(blocks_for(n)), dim3(256), 0, P->stream, P->masks + L.mask_off,
                            P->st[0] + L.st_off, n);
         MXB_BHIP(hipGetLastError());
@@ -331,7 +360,7 @@ static int enqueue_persistent(mxb_batch_plan *P) {
         const mxb_device_volume &V = P->vols[v];
         const int64_t n = L.nz * L.ny * L.nx;
         hipLaunchKernelGGL(k_unpad_out, dim3(blocks_for(n)), dim3(256), 0, P->stream, P->st[parity] + L.st_off,
-                           L.nz, L.ny, L.nx, L.px, V.dev_out_u16, V.dev_out_f32);
+                           L.nz, L.ny, L.nx, L.py, L.px, V.dev_out_u16, V.dev_out_f32);
         MXB_BHIP(hipGetLastError());
     }
     return OK;
@@ -366,7 +395,7 @@ static int autotune(mxb_batch_plan *P, int cus) {
     // init state so that the timed repetitions read finite values
     for (int v = 0; v < P->n; ++v) {
         const VolLayout &L = P->lay[v];
-        const int64_t n = L.nz * L.ny * L.px;
+        const int64_t n = L.nz * L.py * L.px;
         hipLaunchKernelGGL(k_init_pitched, dim3(blocks_for(n)), dim3(256), 0, P->stream, P->masks + L.mask_off,
                            P->st[0] + L.st_off, n);
         MXB_BHIP(hipGetLastError());
@@ -459,17 +488,17 @@ int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, 
         }
         VolLayout L{};
         L.nz = V.nz; L.ny = V.ny; L.nx = V.nx;
-        L.px = round_up(V.nx, 4);
-        L.py = round_up(V.ny, 4);
+        L.px = round_up(V.nx, 16);
+        L.py = round_up(V.ny, 16);
         L.R = (int)(V.kernel_size / 2);
         if (L.R > 64) P->persistent = false;
         L.bucket = kNumBuckets - 1;
         for (int b = 0; b < kNumBuckets; ++b)
             if (L.R <= kBucketRT[b]) { L.bucket = b; break; }
         L.st_off = st_total;
-        st_total += round_up(std::max(V.nz * V.ny * L.px, V.nz * V.nx * L.py), 64);
+        st_total += round_up(V.nz * L.py * L.px, 64);
         L.mask_off = mask_total;
-        mask_total += round_up(V.nz * V.ny * L.px, 256);
+        mask_total += round_up(V.nz * L.py * L.px, 256);
         P->lay.push_back(L);
         P->host_taps.emplace_back(V.kernel, V.kernel + V.kernel_size);
         key += "_" + std::to_string(V.nz) + "x" + std::to_string(V.ny) + "x" + std::to_string(V.nx) + "r" +
@@ -485,7 +514,7 @@ int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, 
     for (auto &L : P->lay) {
         // an upper bound on items of any pass for any segment choice (seg >= 128)
         const int64_t m = std::max({L.nz, L.ny, L.nx});
-        items_cap += 3 * ((std::max(L.nx, L.ny) + 63) / 64) * std::max(L.ny, L.nz) * ((m + 127) / 128);
+        items_cap += 3 * ((std::max(L.px, L.py) + 63) / 64) * std::max(L.ny, L.nz) * ((m + 127) / 128);
     }
     P->item_cap = (int)std::min<int64_t>(items_cap, INT32_MAX / 2);
     const int64_t taps_cap = 2 * 65 * (int64_t)n + 16;
@@ -526,17 +555,19 @@ int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, 
     }
     for (int v = 0; v < n; ++v) {
         const VolLayout &L = P->lay[v];
-        e = hipMemcpy2DAsync(P->masks + L.mask_off, (size_t)L.px, vols[v].dev_mask, (size_t)L.nx, (size_t)L.nx,
-                             (size_t)(L.nz * L.ny), hipMemcpyDeviceToDevice, P->stream);
-        if (e != hipSuccess) {
-            set_error(std::string("hipMemcpy2DAsync: ") + hipGetErrorString(e));
+        hipLaunchKernelGGL(k_pitch_mask, dim3(blocks_for(L.nz * L.ny * L.nx)), dim3(256), 0, P->stream,
+                           vols[v].dev_mask, L.nz, L.ny, L.nx, L.py, L.px, P->masks + L.mask_off);
+        if ((e = hipGetLastError()) != hipSuccess) {
+            set_error(std::string("k_pitch_mask: ") + hipGetErrorString(e));
             return fail(EDEVICE);
         }
     }
     if (P->persistent) {
-        int smem = 0;
-        for (auto &L : P->lay) smem = std::max(smem, smem_floats(L.bucket) * 4);
-        P->smem_bytes = smem;
+        for (auto &L : P->lay) {
+            P->smem_bytes[0] = std::max(P->smem_bytes[0], smem_bytes_of<0>(L.bucket));
+            P->smem_bytes[1] = std::max(P->smem_bytes[1], smem_bytes_of<1>(L.bucket));
+            P->smem_bytes[2] = std::max(P->smem_bytes[2], smem_bytes_of<2>(L.bucket));
+        }
         int dev = 0, cus = 256;
         if (hipGetDevice(&dev) == hipSuccess) {
             hipDeviceProp_t prop;

@@ -23,21 +23,35 @@ typedef float v4f32 __attribute__((ext_vector_type(4)));
 
 // 16-byte global store; NT: non-temporal (streaming) policy.  Measured on
 // MI355X: it helps passes that write whole 256-byte rows and hurts those that
-// write 64-128-byte pieces, which rely on L2 write combining.
+// write 64-128-byte pieces, which rely on L2 write combining.  The pointer is
+// taken as global memory explicitly: where it comes from a descriptor in
+// memory the compiler could not infer that and would emit a flat store, which
+// also counts against the LDS wait counter.
+typedef __attribute__((address_space(1))) v4u32 g_v4u32;
+typedef __attribute__((address_space(1))) v4f32 g_v4f32;
+
+// `p` as a global-memory pointer (see store16).
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p) {
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 template <bool NT>
 __device__ __forceinline__ void store16(void *p, uint4 v) {
     const v4u32 x = {v.x, v.y, v.z, v.w};
+    g_v4u32 *q = (g_v4u32 *)p;
     if (NT)
-        __builtin_nontemporal_store(x, reinterpret_cast<v4u32 *>(p));
+        __builtin_nontemporal_store(x, q);
     else
-        *reinterpret_cast<v4u32 *>(p) = x;
+        *q = x;
 }
 template <bool NT>
 __device__ __forceinline__ void store16(float *p, v4f32 v) {
+    g_v4f32 *q = (g_v4f32 *)p;
     if (NT)
-        __builtin_nontemporal_store(v, reinterpret_cast<v4f32 *>(p));
+        __builtin_nontemporal_store(v, q);
     else
-        *reinterpret_cast<v4f32 *>(p) = v;
+        *q = v;
 }
 
 // LDS byte address of a __shared__ object.

@@ -1,7 +1,6 @@
-// FP32 line pass, LDS-staged form: single-volume kernel and launcher (the
-// per-block body lives in mxb_lds_body.h).
-#include "mxb_dma.h"
-#include "mxb_lds_body.h"
+// FP32 line pass, LDS-staged forms: single-volume kernels and launcher (the
+// per-block bodies live in mxb_lds_body.h and mxb_f32_dma_body.h).
+#include "mxb_f32_dma_body.h"
 
 namespace mxb {
 
@@ -19,209 +18,18 @@ __global__ __launch_bounds__(256) void k_line_mfma_lds(const float *__restrict__
     lds_line_body<RT>(in, g, e, outer, lb, seg_id, smem);
 }
 
-// ---------------------------------------------------------------------------
-// Direct-load form: the same pass and the same LDS row layout, with each slot
-// (16 positions x 64 lines, 4 KB) and, for the illuminating pass, each
-// 64-line x 16-position mask tile moved global -> LDS by
-// global_load_lds_dwordx4, D slots ahead of use, with no VGPR staging.
-// Lane i of wave w writes LDS bytes 1024 w + 16 i of its slot: row
-// r = 4w + i/16 at float column 4 (i % 16), which holds lines
-// (4 (i % 16) - 16 (r mod 4)) mod 64 .. +3 of the rotated row.
-//
-// Iteration j (tile j): A. wait until group j has landed, barrier;
-// B. issue group j + D into the slots freed by tile j - 1; C. compute tile j;
-// D. stage the outputs in LDS, barrier, S stores per thread.  Group i = data
-// slot i + W - 1 (+ mask tile i); the wait count is (D-1) G + D S once j >= D
-// and (D-1) G before (G loads per group), as in the 16-bit direct-load pass.
-template <int RT, int D>
-struct F32DmaShape {
-    static constexpr int C = LdsShape<RT>::C;
-    static constexpr int W = LdsShape<RT>::W;
-    static constexpr int NSL = W + D;  // ring slots
-    static constexpr int NM = D + 2;   // mask tile slots (a pair's first tile is read at its flush)
-};
-
-constexpr int kF32Slot = 16 * 64;        // floats per slot
-constexpr int kF32StagePitchV = 36;      // floats per line row (position-contiguous epilogue, 2 tiles)
-
 template <int RT, int D, bool MASK, bool OUTF, bool OUTU, bool ZSTAGE>
 __global__ __launch_bounds__(256) void k_line_f32_dma(const float *__restrict__ in, LineGeom g,
                                                       Epilogue e, TapsF<2 * RT + 1> taps) {
-    using S = F32DmaShape<RT, D>;
-    constexpr int C = S::C, W = S::W, NSL = S::NSL, NM = S::NM;
-    constexpr int G = MASK ? 2 : 1;
-    // stores per thread: every tile (line-contiguous) or every second tile
-    // (position-contiguous: a pair of tiles makes 128-byte line pieces)
-    constexpr int ST = ZSTAGE ? 1 : (OUTF ? 2 : 0) + (OUTU ? 1 : 0);
-    static_assert(ZSTAGE || D % 2 == 0, "pair flushes keep the store count fixed only for even D");
-    static_assert(!ZSTAGE || (OUTF && !OUTU && !MASK), "line-contiguous epilogue writes floats only");
-    static_assert(ST >= 1, "at least one output");
-    constexpr int TAPS = 4 * C + 32;
-    constexpr int STAGE = ZSTAGE ? 16 * 64 : 64 * kF32StagePitchV;
-    __shared__ __attribute__((aligned(16))) float ring[NSL * kF32Slot];
-    __shared__ __attribute__((aligned(16))) unsigned char mring[MASK ? NM * 1024 : 16];
-    __shared__ __attribute__((aligned(16))) float stage[STAGE];
-    __shared__ float tap_lds[TAPS];
-
+    __shared__ __attribute__((aligned(16))) unsigned char smem[F32DmaSmem<RT, D, MASK, ZSTAGE>::BYTES];
     int64_t bid = blockIdx.x;
     const int64_t seg_id = bid % g.nseg;
     bid /= g.nseg;
     const int64_t nlb = (g.n_lane + 63) >> 6;
     const int64_t lb = bid % nlb;
     const int64_t outer = bid / nlb;
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int col = lane & 15, kg = lane >> 4;
-    const int64_t line0 = lb * 64;
-    const int64_t a0 = seg_id * g.seg;
-    const int64_t a1 = a0 + g.seg < g.n_ax ? a0 + g.seg : g.n_ax;
-    const int64_t n_ax = g.n_ax;
-    const int ntiles = (int)((a1 - a0 + 15) / 16);
-    const int nslots = ntiles + W - 1;
-
-    for (int i = tid; i < TAPS; i += 256) {
-        const int t = i - 16;
-        tap_lds[i] = (t >= 0 && t <= 2 * RT) ? taps.k[t] : 0.0f;
-    }
-
-    // ---- data loads
-    const int dr = 4 * wv + (lane >> 4);
-    const int dl = (4 * (lane & 15) - 16 * (dr & 3)) & 63;
-    const bool dl_ok = line0 + dl < g.n_lane;  // n_lane is a multiple of 4
-    const char *dsrc = reinterpret_cast<const char *>(in + outer * g.in_outer + line0 + dl);
-    const int64_t in_ax_b = g.in_ax * 4;
-    const int64_t pos_first = a0 - RT;
-    const uint32_t ring_lds = lds_addr(ring) + 1024u * (uint32_t)wv;
-    auto issue_data = [&](int slot) {
-        const int64_t p = pos_first + 16 * (int64_t)slot + dr;
-        const bool ok = dl_ok && slot < nslots && p >= 0 && p < n_ax;
-        const void *src = ok ? static_cast<const void *>(dsrc + p * in_ax_b)
-                             : static_cast<const void *>(g_dma_zero);
-        dma16<ZSTAGE>(ring_lds + (uint32_t)((slot % NSL) * kF32Slot * 4), src);
-    };
-    // ---- mask loads (lanes < 16): line 16 wv + lane, 16 positions
-    const bool ml_ok = MASK && line0 + 16 * wv + lane < g.n_lane;
-    const uint8_t *msrc = MASK ? e.mask + outer * g.out_outer + (line0 + 16 * wv + lane) * g.out_lane : nullptr;
-    const uint32_t mring_lds = lds_addr(mring) + 256u * (uint32_t)wv;
-    auto issue_mask = [&](int tj) {
-        const int64_t tpos = a0 + 16 * (int64_t)tj;
-        const bool ok = ml_ok && tpos + 16 <= a1;
-        const void *src = ok ? static_cast<const void *>(msrc + tpos) : static_cast<const void *>(g_dma_zero);
-        if (lane < 16) dma16<false>(mring_lds + (uint32_t)((tj % NM) * 1024), src);
-    };
-    auto issue_group = [&](int i) {
-        issue_data(i + W - 1);
-        if (MASK) issue_mask(i);
-    };
-
-    for (int s = 0; s < W - 1; ++s) issue_data(s);
-    for (int i = 0; i < D; ++i) issue_group(i);
-    __syncthreads();  // tap table
-
-    float tv[C];
-#pragma unroll
-    for (int c = 0; c < C; ++c) tv[c] = tap_lds[4 * c + kg - col + 16];
-    const int rd_lane = kg * 64 + ((16 * wv + col + 16 * kg) & 63);
-
-    // store-side constants
-    const int zr = tid >> 4, zq = tid & 15;  // line-contiguous: position, 4 lines
-    const bool zq_ok = line0 + 4 * zq < g.n_lane;
-    const int sl = tid >> 2, sq = tid & 3;   // position-contiguous: line, 8 positions
-    const bool sl_ok = line0 + sl < g.n_lane;
-    float *const sinkf = reinterpret_cast<float *>(g_dma_sink + lane);
-    uint16_t *const sinku = reinterpret_cast<uint16_t *>(g_dma_sink + lane);
-
-    for (int j = 0; j < ntiles; ++j) {
-        if (j >= D)
-            vm_wait<(D - 1) * G + (ZSTAGE ? D : D / 2) * ST>();
-        else
-            vm_wait<(D - 1) * G>();
-        __syncthreads();
-        issue_group(j + D);
-        int sb[W];
-#pragma unroll
-        for (int s2 = 0; s2 < W; ++s2) sb[s2] = ((j + s2) % NSL) * kF32Slot;
-        float x[C];
-#pragma unroll
-        for (int c = 0; c < C; ++c) x[c] = ring[sb[c / 4] + (c % 4) * 256 + rd_lane];
-        // one accumulation chain in tap order (bit-identical to the
-        // one-FMA-per-tap form, whatever the tile alignment)
-        v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int c = 0; c < C; ++c) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(tv[c], x[c], acc, 0, 0, 0);
-        const int64_t tpos = a0 + 16 * (int64_t)j;
-        const bool full = tpos + 16 <= a1;  // uniform; only a segment's last tile is partial
-        if (ZSTAGE) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) stage[(4 * kg + r) * 64 + 16 * wv + col] = acc[r];
-            __syncthreads();
-            const v4f v = *reinterpret_cast<const v4f *>(&stage[zr * 64 + 4 * zq]);
-            const int64_t p = tpos + zr;
-            float *op = e.out_f32 + outer * g.out_outer + p * g.out_ax + line0 + 4 * zq;
-            if (full) {
-                store16<true>(zq_ok ? op : sinkf, v);
-            } else if (zq_ok && p < a1) {
-                *reinterpret_cast<v4f *>(op) = v;
-            }
-        } else {
-            const int half = j & 1;
-            *reinterpret_cast<v4f *>(&stage[(16 * wv + col) * kF32StagePitchV + 16 * half + 4 * kg]) = acc;
-            if (half == 1 || j == ntiles - 1) {
-                __syncthreads();
-                const int64_t ppos = tpos - 16 * half;  // first position of the pair
-                const bool pfull = ppos + 32 <= a1;     // uniform
-                v4f v[2];
-#pragma unroll
-                for (int h = 0; h < 2; ++h)
-                    v[h] = *reinterpret_cast<const v4f *>(&stage[sl * kF32StagePitchV + 8 * sq + 4 * h]);
-                const int64_t p0 = ppos + 8 * sq;
-                const int64_t off = outer * g.out_outer + (line0 + sl) * g.out_lane + p0;
-                if (pfull && half == 1) {
-                    if (MASK) {
-                        // positions 8 sq .. +7 of the pair: tile j-1 for sq < 2, else tile j
-                        const int tj = sq < 2 ? j - 1 : j;
-                        const uint2 m = *reinterpret_cast<const uint2 *>(mring + (tj % NM) * 1024 + sl * 16 + 8 * (sq & 1));
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            if ((m.x >> (8 * r)) & 0xffu) v[0][r] = 1.0f;
-                            if ((m.y >> (8 * r)) & 0xffu) v[1][r] = 1.0f;
-                        }
-                    }
-                    if (OUTF) {
-                        float *dst = sl_ok ? e.out_f32 + off : sinkf;
-                        store16<false>(dst, v[0]);
-                        store16<false>(dst + (sl_ok ? 4 : 0), v[1]);
-                    }
-                    if (OUTU) {
-                        uint32_t w[4];
-#pragma unroll
-                        for (int h = 0; h < 2; ++h) {
-                            w[2 * h] = (uint32_t)quantise_u16_l(v[h][0]) | ((uint32_t)quantise_u16_l(v[h][1]) << 16);
-                            w[2 * h + 1] = (uint32_t)quantise_u16_l(v[h][2]) | ((uint32_t)quantise_u16_l(v[h][3]) << 16);
-                        }
-                        store16<false>(sl_ok ? e.out_u16 + off : sinku, make_uint4(w[0], w[1], w[2], w[3]));
-                    }
-                } else if (sl_ok) {
-                    // a segment's last (partial or unpaired) tiles
-#pragma unroll
-                    for (int r = 0; r < 8; ++r) {
-                        if (p0 + r >= a1 || p0 + r >= tpos + 16) break;
-                        float w = v[r >> 2][r & 3];
-                        if (MASK && e.mask[off + r] != 0) w = 1.0f;
-                        if (OUTF) e.out_f32[off + r] = w;
-                        if (OUTU) e.out_u16[off + r] = quantise_u16_l(w);
-                    }
-                }
-            }
-        }
-    }
+    f32_dma_body<RT, D, MASK, OUTF, OUTU, ZSTAGE>(in, g, e, outer, lb, seg_id, taps.k, smem);
 }
-
-#ifndef MXB_F32_DMA_DEPTH
-#define MXB_F32_DMA_DEPTH 2
-#endif
 
 // Which direct-load epilogue serves (g, e): 0 none (use the VGPR-staged
 // form), else a code for the template below.
