@@ -1,4 +1,5 @@
-"""ctypes bindings to the C ABI of libmxb_diffusion (include/maxibone_diffusion.h).
+"""ctypes bindings to the C ABI of libmxb_diffusion (include/maxibone_diffusion.h,
+include/maxibone_fields.h).
 
 The library is loaded from the package directory, where ``maxibone_amd.build``
 places it.  There is no fallback: if the library is missing, ``lib()`` raises.
@@ -11,7 +12,8 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libmxb_diffusion.so")
-HEADER = os.path.join(os.path.dirname(PKG), "include", "maxibone_diffusion.h")
+HEADERS = [os.path.join(os.path.dirname(PKG), "include", h)
+           for h in ("maxibone_diffusion.h", "maxibone_fields.h")]
 
 OK, EINVAL, EDEVICE, ENOMEM, EIO = 0, 1, 2, 3, 4
 PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "fp32_fma": 3, "fp32_exact": 4}
@@ -62,6 +64,20 @@ def lib():
             "mxb_batch_info": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)],
             "mxb_batch_destroy": [_vp],
+            "mxb_combine_fields": [_vp, _vp, _i64, _i64, _i64, _vp, _vp, ctypes.c_int],
+            "mxb_device_combine_fields": [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp],
+            "mxb_field_histogram": [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _vp, _i64, _i64,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int],
+            "mxb_device_field_histogram": [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _vp, _i64,
+                                           _i64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, _vp, _vp],
+            "mxb_axis_histograms": [_vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _i64,
+                                    ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int],
+            "mxb_device_axis_histograms": [_vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _i64,
+                                           _i64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double,
+                                           ctypes.c_double, _vp, _vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -69,6 +85,10 @@ def lib():
             f.restype = ctypes.c_int
         L.mxb_state_bytes.argtypes = [ctypes.c_int, _i64, _i64, _i64]
         L.mxb_state_bytes.restype = _i64
+        L.mxb_combine_workspace_bytes.argtypes = []
+        L.mxb_combine_workspace_bytes.restype = _i64
+        L.mxb_histogram_workspace_bytes.argtypes = [_i64, _i64, _i64]
+        L.mxb_histogram_workspace_bytes.restype = _i64
         L.mxb_last_error.restype = ctypes.c_char_p
         L.mxb_version.restype = ctypes.c_char_p
         _LIB = L
@@ -92,10 +112,13 @@ def check(rc):
 
 
 def declared_functions():
-    """Function names declared in include/maxibone_diffusion.h."""
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z0-9_]+\s+\*?\s*(mxb_[a-z0-9_]+)\s*\(", text,
-                                 flags=re.M)))
+    """Function names declared in include/*.h."""
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        names |= set(re.findall(r"^\s*(?:const\s+)?[a-z0-9_]+\s+\*?\s*(mxb_[a-z0-9_]+)\s*\(", text,
+                                flags=re.M))
+    return sorted(names)
 
 
 def device_count():

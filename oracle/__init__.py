@@ -143,13 +143,14 @@ def quantise_u8(u):
     return q
 
 
-def ref_module():
-    """The reference CPU module built from source into oracle/_ref, or None."""
+def ref_module(name="diffusion"):
+    """A reference CPU module (diffusion, histograms, general) built from
+    source into oracle/_ref, or None."""
     from .build import ref_module_path
-    path = ref_module_path()
+    path = ref_module_path(name)
     if not os.path.exists(path):
         return None
-    spec = importlib.util.spec_from_file_location("diffusion", path)
+    spec = importlib.util.spec_from_file_location(name, path)
     mod = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mod)
     return mod

@@ -3,7 +3,7 @@
 * ``liboracle.so``: the plain-C restatement in ``diffusion_oracle.c``.
   Flags follow the reference CPU build (src/Makefile:9-18: -O3, OpenMP, no
   -march) plus ``-ffp-contract=off`` so that no FMA is ever formed.
-* ``_ref/diffusion<ext-suffix>``: the reference's own CPU module, compiled
+* ``_ref/<module><ext-suffix>``: the reference's own CPU modules, compiled
   directly from its sources where they lie
   (src/pybind/diffusion-pybind.cc, which #includes
   src/lib/cpp/cpu/diffusion.cc and src/lib/cpp/include/*.hh) with the
@@ -35,19 +35,26 @@ def build_oracle() -> str:
     return out
 
 
-def ref_module_path() -> str:
+def ref_module_path(name: str = "diffusion") -> str:
     suffix = sysconfig.get_config_var("EXT_SUFFIX")
-    return os.path.join(HERE, "_ref", "diffusion" + suffix)
+    return os.path.join(HERE, "_ref", name + suffix)
 
 
-def build_ref() -> str | None:
-    """Compile the reference CPU diffusion module from its own sources."""
-    pyb = os.path.join(REF_SRC, "pybind", "diffusion-pybind.cc")
+# Reference CPU modules compiled from their own sources: the diffusion path and
+# its downstream consumers (histograms: src/pybind/histograms-pybind.cc with
+# src/lib/cpp/cpu/histograms.cc; general: src/pybind/general-pybind.cc with
+# src/lib/cpp/cpu/general.cc, for the C++ to_int / normalized_convert).
+REF_MODULES = ("diffusion", "histograms", "general")
+
+
+def build_ref(name: str = "diffusion") -> str | None:
+    """Compile the reference CPU module `name` from its own sources."""
+    pyb = os.path.join(REF_SRC, "pybind", name + "-pybind.cc")
     if not os.path.exists(pyb):
         return None
     import pybind11  # header-only, part of the image
 
-    out = ref_module_path()
+    out = ref_module_path(name)
     os.makedirs(os.path.dirname(out), exist_ok=True)
     inc = [
         "-isystem", pybind11.get_include(),
@@ -65,5 +72,6 @@ def build_ref() -> str | None:
 
 if __name__ == "__main__":
     print(build_oracle())
-    print(build_ref())
+    for m in REF_MODULES:
+        print(build_ref(m))
     sys.exit(0)
