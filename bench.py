@@ -35,7 +35,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "batch"])
+    p.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "batch", "fields"])
     p.add_argument("--precision", default="fp32",
                    choices=["fp32", "bf16", "fp16", "fp32_fma", "fp32_exact"])
     p.add_argument("--reps", type=int, default=None, help="override the workload's repetitions")
@@ -156,6 +156,147 @@ def bench_batch(args):
     print(json.dumps(rec), flush=True)
 
 
+def synth_ct(shape, seed):
+    """uint16 CT-like volumes for the field consumers, generated slice by slice:
+    voxels = material classes by radius (implant, bone, soft tissue, air) with
+    noise; gauss = a smooth field decaying with the distance from the axis;
+    edt = the distance from the implant surface."""
+    nz, ny, nx = shape
+    rng = np.random.default_rng(seed)
+    yy, xx = np.meshgrid(np.arange(ny) - ny / 2, np.arange(nx) - nx / 2, indexing="ij")
+    r = np.hypot(yy, xx) / (min(ny, nx) / 2)
+    base = np.select([r < 0.15, r < 0.45, r < 0.9], [52000.0, 24000.0, 9000.0], 1500.0)
+    gauss2 = np.clip(65535.0 * np.exp(-np.maximum(r - 0.15, 0) * 6), 0, 65535)
+    edt2 = np.clip(np.abs(r - 0.15) * 40000.0, 0, 65535)
+    vox = np.empty(shape, np.uint16)
+    gauss = np.empty(shape, np.uint16)
+    edt = np.empty(shape, np.uint16)
+    for z in range(nz):
+        t = 1.0 + 0.1 * np.sin(z / 37.0)
+        vox[z] = np.clip(base + rng.normal(0.0, 700.0, (ny, nx)), 0, 65535).astype(np.uint16)
+        gauss[z] = np.clip(gauss2 * t, 0, 65535).astype(np.uint16)
+        edt[z] = edt2.astype(np.uint16)
+    return vox, gauss, edt
+
+
+def bench_fields(args):
+    """The consumers of the diffusion field on 1024^3 uint16 volumes resident in
+    HBM: the 0975 combination (gauss, edt -> uint16), the field histogram of the
+    voxels against the half-resolution Gaussian field (4096 x 2048 bins) and
+    the axis histograms (4096 bins).  One step = the three operations once."""
+    from maxibone_amd import capi, hip
+    from maxibone_amd.helpers import generate_cylinder_mask
+
+    L = capi.lib()
+    shape = (1024, 1024, 1024)
+    nz, ny, nx = shape
+    n = nz * ny * nx
+    vox, gauss, edt = synth_ct(shape, 2024)
+    fld = np.ascontiguousarray(gauss[::2, ::2, ::2])
+    cyl = generate_cylinder_mask(ny, nx).astype(np.uint8)
+    vb, fbins, vrange, frange = 4096, 2048, (1.0, 40000.0), (1.0, 65535.0)
+    d_vox, d_g, d_e, d_f, d_cyl = (hip.DeviceBuffer.from_array(a) for a in (vox, gauss, edt, fld, cyl))
+    d_out = hip.DeviceBuffer(2 * n)
+    d_bins = hip.DeviceBuffer(8 * fbins * vb)
+    d_ab = [hip.DeviceBuffer(8 * m * vb) for m in (nx, ny, nz, 725)]
+    ws_c = hip.DeviceBuffer(L.mxb_combine_workspace_bytes())
+    ws_h = hip.DeviceBuffer(L.mxb_histogram_workspace_bytes(*shape))
+    for b in [d_bins] + d_ab:
+        b.fill_bytes(0)
+    st = hip.Stream()
+    ops = {
+        "combine": lambda: L.mxb_device_combine_fields(d_g.ptr, d_e.ptr, *shape, d_cyl.ptr, d_out.ptr, ws_c.ptr,
+                                                       st.handle),
+        "field_histogram": lambda: L.mxb_device_field_histogram(d_vox.ptr, *shape, d_f.ptr, *fld.shape,
+                                                                d_bins.ptr, fbins, vb, *vrange, *frange,
+                                                                ws_h.ptr, st.handle),
+        "axis_histograms": lambda: L.mxb_device_axis_histograms(d_vox.ptr, *shape, 0, *(b.ptr for b in d_ab), nz,
+                                                                725, vb, ny // 2, nx // 2, *vrange, ws_h.ptr,
+                                                                st.handle),
+    }
+    # algorithmic HBM bytes per voxel: combine reads two uint16 fields and
+    # writes one; the field histogram reads the voxel and 1/8 of a field value
+    # per voxel; the axis histograms read the voxel
+    alg = {"combine": 6.0, "field_histogram": 2.25, "axis_histograms": 2.0}
+    for name, f in ops.items():
+        for _ in range(args.warmup):
+            capi.check(f())
+    st.synchronize()
+    per = {}
+    for name, f in ops.items():
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(st)
+        for _ in range(args.steps):
+            capi.check(f())
+        e1.record(st)
+        ms = e0.elapsed_ms(e1) / args.steps
+        per[name] = {"ms": round(ms, 4), "Gvoxel_per_s": round(n / ms / 1e6, 2),
+                     "algorithmic_GBps": round(alg[name] * n / ms / 1e6, 1)}
+    total_ms = sum(v["ms"] for v in per.values())
+    dom = max(per, key=lambda k: per[k]["ms"])
+    rec = {
+        "metric": "Gvoxel/s (field consumers: combine + field histogram + axis histograms)",
+        "value": round(n / total_ms / 1e6, 3),
+        "unit": "Gvoxel/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(total_ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u16", "data": "synthetic",
+        "config": {"workload": "1024^3 uint16 voxels, Gaussian and EDT fields; field histogram against the "
+                               "512^3 field, 4096 voxel x 2048 field bins; axis histograms 4096 bins",
+                   "shape": list(shape)},
+        "per_op": per,
+        "roofline": {"bound": "hbm", "achieved": per[dom]["algorithmic_GBps"], "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(per[dom]["algorithmic_GBps"] / PEAK_HBM_GBS, 4),
+                     "traffic": None, "kernel": dom},
+    }
+    for b in (d_vox, d_g, d_e, d_f, d_cyl, d_out, d_bins, ws_c, ws_h, *d_ab):
+        b.free()
+    if args.cpu_seconds > 0:
+        rec["cpu_baseline"] = fields_cpu_baseline(vox, gauss, edt, fld, cyl, vb, fbins, vrange, frange)
+    else:
+        rec["cpu_baseline"] = None
+    print(json.dumps(rec), flush=True)
+
+
+def fields_cpu_baseline(vox, gauss, edt, fld, cyl, vb, fbins, vrange, frange):
+    """The reference's CPU forms on a 64-layer slab of the same volumes: the
+    histograms module compiled from its sources (OpenMP, all host cores) and
+    the numpy combination of 0975 with the compiled to_int."""
+    import oracle
+    H = oracle.ref_module("histograms")
+    G = oracle.ref_module("general")
+    if H is None or G is None:
+        return None
+    slab = 64
+    v, g, e, f = vox[:slab], gauss[:slab], edt[:slab], fld[:slab // 2]
+    n = v.size
+    t = {}
+    t0 = time.perf_counter()
+    c = g.astype(np.float32) / g.max()
+    c -= e.astype(np.float32) / e.max()
+    c += np.abs(c.min())
+    c /= c.max()
+    c *= cyl.astype(bool)
+    q = np.empty(c.shape, np.uint16)
+    G.normalized_convert(c, q)
+    q *= cyl.astype(bool)
+    t["combine"] = time.perf_counter() - t0
+    bins = np.zeros((fbins, vb), np.uint64)
+    t0 = time.perf_counter()
+    H.field_histogram(v, f, (0, 0, 0), bins, vrange, frange, 0)
+    t["field_histogram"] = time.perf_counter() - t0
+    ab = [np.zeros((m, vb), np.uint64) for m in (v.shape[2], v.shape[1], v.shape[0], 725)]
+    t0 = time.perf_counter()
+    H.axis_histograms(v, (0, 0, 0), *ab, (v.shape[1] // 2, v.shape[2] // 2), vrange, 0)
+    t["axis_histograms"] = time.perf_counter() - t0
+    total = sum(t.values())
+    return {"value": round(n / total / 1e9, 4), "unit": "Gvoxel/s", "cores": os.cpu_count(),
+            "kind": "reference",
+            "sample": f"64 x 1024 x 1024 slab: combine {t['combine']:.2f} s (numpy + compiled to_int), "
+                      f"field histogram {t['field_histogram']:.2f} s, axis histograms "
+                      f"{t['axis_histograms']:.2f} s (compiled CPU histograms module)"}
+
+
 def load_traffic(workload_name, precision, kernel):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -178,6 +319,9 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.workload == "fields":
+        bench_fields(args)
         return
     from maxibone_amd import capi, hip
     from maxibone_amd.synth import WORKLOADS

@@ -20,6 +20,7 @@
 // tables (-1 = not counted); field coordinates floor(Z*nz/nZ) etc. become
 // per-axis maps.  Kernels add with 64-bit global atomics (vector memory
 // atomics), after which the caller's bins hold the sums.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -91,10 +92,12 @@ __global__ __launch_bounds__(64) void k_fields_init(uint32_t *ws) {
 __global__ __launch_bounds__(kRowThreads) void k_max2(const uint16_t *__restrict__ a,
                                                       const uint16_t *__restrict__ b, RowWalk w,
                                                       int vec, uint32_t *ws) {
-    const int64_t row = blockIdx.x / w.chunks;
-    const int64_t x0 = (blockIdx.x % w.chunks) * kRowChunk + threadIdx.x * kPerThread;
     uint32_t ma = 0, mb = 0;
-    if (row < w.rows && x0 < w.nx) {
+    // grid-stride over (row, chunk): one atomic per block at the end
+    for (int64_t i = blockIdx.x; i < w.rows * w.chunks; i += gridDim.x) {
+        const int64_t row = i / w.chunks;
+        const int64_t x0 = (i % w.chunks) * kRowChunk + threadIdx.x * kPerThread;
+        if (x0 >= w.nx) continue;
         uint16_t va[8], vb[8];
         load8(a + row * w.nx, x0, w.nx, vec != 0, va);
         load8(b + row * w.nx, x0, w.nx, vec != 0, vb);
@@ -132,10 +135,11 @@ __global__ __launch_bounds__(kRowThreads) void k_combine_minmax(const uint16_t *
                                                                 const uint8_t *__restrict__ cyl,
                                                                 RowWalk w, int64_t ny, int vec,
                                                                 float gm, float em, uint32_t *ws) {
-    const int64_t row = blockIdx.x / w.chunks;
-    const int64_t x0 = (blockIdx.x % w.chunks) * kRowChunk + threadIdx.x * kPerThread;
     uint32_t lo = 0xffffffffu, hi = 0u, lo_c = 0xffffffffu, hi_c = 0u;
-    if (row < w.rows && x0 < w.nx) {
+    for (int64_t i = blockIdx.x; i < w.rows * w.chunks; i += gridDim.x) {
+        const int64_t row = i / w.chunks;
+        const int64_t x0 = (i % w.chunks) * kRowChunk + threadIdx.x * kPerThread;
+        if (x0 >= w.nx) continue;
         uint16_t vg[8], ve[8];
         load8(g + row * w.nx, x0, w.nx, vec != 0, vg);
         load8(e + row * w.nx, x0, w.nx, vec != 0, ve);
@@ -216,25 +220,69 @@ struct FieldHistArgs {
     const int32_t *zmap, *ymap, *xmap;   // voxel coordinate -> field coordinate
 };
 
+// Field histogram.  A block takes kFlushItems consecutive (row, 1024-voxel
+// chunk) items at a time (adjacent rows: similar field values and material
+// classes), gathers their (field bin, voxel bin) pairs in an LDS hash table of
+// kHashSlots keys (linear probing; at most kFlushItems * kRowChunk distinct
+// keys per flush, half the table, so a free slot or the key itself is always
+// found), then adds each distinct pair to the global bins with one atomic.
+constexpr int kFlushItems = 4;
+constexpr int kHashSlots = 2 * kFlushItems * kRowChunk;  // 8192
+constexpr int kHashShift = 19;                            // 32 - log2(kHashSlots)
+constexpr uint32_t kEmpty = 0xffffffffu;
+static_assert(kHashSlots == (1 << (32 - kHashShift)), "hash table size");
+
 __global__ __launch_bounds__(kRowThreads) void k_field_histogram(const uint16_t *__restrict__ vox,
                                                                  const uint16_t *__restrict__ field,
                                                                  RowWalk w, FieldHistArgs a, int vec,
                                                                  unsigned long long *__restrict__ bins) {
-    const int64_t row = blockIdx.x / w.chunks;
-    const int64_t x0 = (blockIdx.x % w.chunks) * kRowChunk + threadIdx.x * kPerThread;
-    if (row >= w.rows || x0 >= w.nx) return;
-    uint16_t v[8];
-    load8(vox + row * w.nx, x0, w.nx, vec != 0, v);
-    const int64_t Y = row % a.nY, Z = row / a.nY;
-    const uint16_t *frow = field + ((int64_t)a.zmap[Z] * a.ny + a.ymap[Y]) * a.nx;
+    __shared__ uint32_t keys[kHashSlots];
+    __shared__ uint32_t counts[kHashSlots];
+    for (int i = threadIdx.x; i < kHashSlots; i += kRowThreads) {
+        keys[i] = kEmpty;
+        counts[i] = 0;
+    }
+    __syncthreads();
+    const int64_t items = w.rows * w.chunks;
+    for (int64_t first = (int64_t)blockIdx.x * kFlushItems; first < items;
+         first += (int64_t)gridDim.x * kFlushItems) {
+        for (int64_t item = first; item < first + kFlushItems && item < items; ++item) {
+            const int64_t row = item / w.chunks;
+            const int64_t x0 = (item % w.chunks) * kRowChunk + threadIdx.x * kPerThread;
+            if (x0 >= w.nx) continue;
+            uint16_t v[8];
+            load8(vox + row * w.nx, x0, w.nx, vec != 0, v);
+            const int64_t Y = row % a.nY, Z = row / a.nY;
+            const uint16_t *frow = field + ((int64_t)a.zmap[Z] * a.ny + a.ymap[Y]) * a.nx;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        if (x0 + k >= w.nx) break;
-        const int32_t vi = a.vlut[v[k]];
-        if (vi < 0) continue;
-        const int32_t fi = a.flut[frow[a.xmap[x0 + k]]];
-        if (fi < 0) continue;
-        atomicAdd(&bins[(int64_t)fi * a.vb + vi], 1ull);
+            for (int k = 0; k < 8; ++k) {
+                if (x0 + k >= w.nx) break;
+                const int32_t vi = a.vlut[v[k]];
+                if (vi < 0) continue;
+                const int32_t fi = a.flut[frow[a.xmap[x0 + k]]];
+                if (fi < 0) continue;
+                const uint32_t key = (uint32_t)fi * (uint32_t)a.vb + (uint32_t)vi;
+                uint32_t slot = (key * 2654435761u) >> kHashShift;
+                while (true) {
+                    const uint32_t prev = atomicCAS(&keys[slot], kEmpty, key);
+                    if (prev == kEmpty || prev == key) {
+                        atomicAdd(&counts[slot], 1u);
+                        break;
+                    }
+                    slot = (slot + 1) & (kHashSlots - 1);
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kHashSlots; i += kRowThreads) {
+            const uint32_t key = keys[i];
+            if (key != kEmpty) {
+                atomicAdd(&bins[key], (unsigned long long)counts[i]);
+                keys[i] = kEmpty;
+                counts[i] = 0;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -251,31 +299,67 @@ __device__ __forceinline__ int64_t isqrt64(int64_t s) {
     return r;
 }
 
+// Axis histograms.  A block walks whole rows (grid-stride); the voxels of a
+// row share y and z, so their bin counts gather in LDS (one histogram for the
+// row's y, one for the current z plane) and reach y_bins / z_bins as one
+// atomic per nonzero bin: at the end of each row for y, when the plane
+// changes (rows of a block come in increasing order) for z.  x and r vary
+// along the row and take one global atomic per voxel.  LDS: 2 x vb uint32.
 __global__ __launch_bounds__(kRowThreads) void k_axis_histograms(const uint16_t *__restrict__ vox,
                                                                  RowWalk w, AxisHistArgs a, int vec,
                                                                  unsigned long long *__restrict__ xb,
                                                                  unsigned long long *__restrict__ yb,
                                                                  unsigned long long *__restrict__ zb,
                                                                  unsigned long long *__restrict__ rb) {
-    const int64_t row = blockIdx.x / w.chunks;
-    const int64_t x0 = (blockIdx.x % w.chunks) * kRowChunk + threadIdx.x * kPerThread;
-    if (row >= w.rows || x0 >= w.nx) return;
-    const int64_t y = row % a.ny, z = row / a.ny + a.z_offset;
-    if (z >= a.Nz) return;
-    uint16_t v[8];
-    load8(vox + row * w.nx, x0, w.nx, vec != 0, v);
-    const int64_t dy2 = (y - a.cy) * (y - a.cy);
+    extern __shared__ uint32_t hist[];  // [0, vb): row (y); [vb, 2 vb): plane (z)
+    uint32_t *hy = hist, *hz = hist + a.vb;
+    for (int64_t i = threadIdx.x; i < 2 * a.vb; i += kRowThreads) hist[i] = 0;
+    __syncthreads();
+    int64_t zcur = -1;  // plane whose counts hz holds (uniform)
+    auto flush = [&](uint32_t *h, unsigned long long *dst) {
+        for (int64_t i = threadIdx.x; i < a.vb; i += kRowThreads) {
+            const uint32_t c = h[i];
+            if (c != 0) {
+                atomicAdd(&dst[i], (unsigned long long)c);
+                h[i] = 0;
+            }
+        }
+    };
+    for (int64_t row = blockIdx.x; row < w.rows; row += gridDim.x) {
+        const int64_t y = row % a.ny, z = row / a.ny + a.z_offset;
+        if (z >= a.Nz) break;  // rows come in increasing order: the rest lie beyond Nz too
+        if (z != zcur) {
+            if (zcur >= 0) {
+                __syncthreads();
+                flush(hz, zb + zcur * a.vb);
+                __syncthreads();
+            }
+            zcur = z;
+        }
+        const int64_t dy2 = (y - a.cy) * (y - a.cy);
+        for (int64_t x0 = (int64_t)threadIdx.x * kPerThread; x0 < w.nx; x0 += kRowChunk) {
+            uint16_t v[8];
+            load8(vox + row * w.nx, x0, w.nx, vec != 0, v);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int64_t x = x0 + k;
-        if (x >= w.nx) break;
-        const int32_t vi = a.vlut[v[k]];
-        if (vi < 0) continue;
-        atomicAdd(&xb[x * a.vb + vi], 1ull);
-        atomicAdd(&yb[y * a.vb + vi], 1ull);
-        atomicAdd(&zb[z * a.vb + vi], 1ull);
-        const int64_t r = isqrt64((x - a.cx) * (x - a.cx) + dy2);
-        if (r < a.Nr) atomicAdd(&rb[r * a.vb + vi], 1ull);
+            for (int k = 0; k < 8; ++k) {
+                const int64_t x = x0 + k;
+                if (x >= w.nx) break;
+                const int32_t vi = a.vlut[v[k]];
+                if (vi < 0) continue;
+                atomicAdd(&xb[x * a.vb + vi], 1ull);
+                const int64_t r = isqrt64((x - a.cx) * (x - a.cx) + dy2);
+                if (r < a.Nr) atomicAdd(&rb[r * a.vb + vi], 1ull);
+                atomicAdd(&hy[vi], 1u);
+                atomicAdd(&hz[vi], 1u);
+            }
+        }
+        __syncthreads();
+        flush(hy, yb + y * a.vb);
+        __syncthreads();
+    }
+    if (zcur >= 0) {
+        __syncthreads();
+        flush(hz, zb + zcur * a.vb);
     }
 }
 
@@ -303,6 +387,11 @@ static int vec_ok(const void *p, int64_t nx) {
 
 static unsigned grid_of(const RowWalk &w) {
     return (unsigned)(w.rows * w.chunks);
+}
+
+// Reductions: a grid-stride grid of at most 8192 blocks (one atomic each).
+static unsigned reduce_grid(const RowWalk &w) {
+    return (unsigned)std::min<int64_t>(w.rows * w.chunks, 8192);
 }
 
 #define MXB_FHIP(expr)                                                          \
@@ -333,7 +422,7 @@ static int device_combine(const uint16_t *g, const uint16_t *e, int64_t nz, int6
     if (int rc = check_grid(w)) return rc;
     const int vec = vec_ok(g, nx) & vec_ok(e, nx) & vec_ok(out, nx);
     hipLaunchKernelGGL(k_fields_init, dim3(1), dim3(64), 0, s, ws);
-    hipLaunchKernelGGL(k_max2, dim3(grid_of(w)), dim3(kRowThreads), 0, s, g, e, w, vec, ws);
+    hipLaunchKernelGGL(k_max2, dim3(reduce_grid(w)), dim3(kRowThreads), 0, s, g, e, w, vec, ws);
     MXB_FHIP(hipGetLastError());
     uint32_t h[6];
     MXB_FHIP(hipMemcpyAsync(h, ws, 8, hipMemcpyDeviceToHost, s));
@@ -343,7 +432,7 @@ static int device_combine(const uint16_t *g, const uint16_t *e, int64_t nz, int6
         return EINVAL_;
     }
     const float gm = (float)h[0], em = (float)h[1];
-    hipLaunchKernelGGL(k_combine_minmax, dim3(grid_of(w)), dim3(kRowThreads), 0, s, g, e, cyl_dev, w, ny,
+    hipLaunchKernelGGL(k_combine_minmax, dim3(reduce_grid(w)), dim3(kRowThreads), 0, s, g, e, cyl_dev, w, ny,
                        vec, gm, em, ws);
     MXB_FHIP(hipGetLastError());
     MXB_FHIP(hipMemcpyAsync(h, ws, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -390,7 +479,7 @@ static int device_field_histogram(const uint16_t *vox, int64_t nZ, int64_t nY, i
                                   uint64_t *bins, int64_t fb, int64_t vb, double vmin, double vmax,
                                   double fmin, double fmax, void *ws, hipStream_t s) {
     if (nZ * nY * nX == 0) return OK;
-    if (nz < 1 || ny < 1 || nx < 1 || fb < 1 || vb < 1 || fb * vb > (int64_t(1) << 40)) {
+    if (nz < 1 || ny < 1 || nx < 1 || fb < 1 || vb < 1 || fb * vb >= (int64_t)0xffffffffLL) {
         set_error("field_histogram: invalid field shape or bin counts");
         return EINVAL_;
     }
@@ -409,7 +498,8 @@ static int device_field_histogram(const uint16_t *vox, int64_t nZ, int64_t nY, i
     MXB_FHIP(hipMemcpyAsync(d, host.data(), host.size() * 4, hipMemcpyHostToDevice, s));
     FieldHistArgs a{nZ, nY, nX, nz, ny, nx, vb, fb, d, d + 65536, d + 2 * 65536, d + 2 * 65536 + nZ,
                     d + 2 * 65536 + nZ + nY};
-    hipLaunchKernelGGL(k_field_histogram, dim3(grid_of(w)), dim3(kRowThreads), 0, s, vox, field, w, a,
+    const unsigned grid = (unsigned)std::min<int64_t>((w.rows * w.chunks + kFlushItems - 1) / kFlushItems, 4096);
+    hipLaunchKernelGGL(k_field_histogram, dim3(grid), dim3(kRowThreads), 0, s, vox, field, w, a,
                        vec_ok(vox, nX), (unsigned long long *)bins);
     MXB_FHIP(hipGetLastError());
     // the host table must outlive the copy
@@ -433,7 +523,12 @@ static int device_axis_histograms(const uint16_t *vox, int64_t nz, int64_t ny, i
     int32_t *d = (int32_t *)ws;
     MXB_FHIP(hipMemcpyAsync(d, host.data(), host.size() * 4, hipMemcpyHostToDevice, s));
     AxisHistArgs a{nz, ny, nx, z_offset, Nz, Nr, vb, d, (int64_t)cy, (int64_t)cx};
-    hipLaunchKernelGGL(k_axis_histograms, dim3(grid_of(w)), dim3(kRowThreads), 0, s, vox, w, a,
+    if (vb > 8192) {
+        set_error("axis_histograms: at most 8192 voxel bins");
+        return EINVAL_;
+    }
+    const unsigned grid = (unsigned)std::min<int64_t>(w.rows, 4096);
+    hipLaunchKernelGGL(k_axis_histograms, dim3(grid), dim3(kRowThreads), (unsigned)(2 * vb * 4), s, vox, w, a,
                        vec_ok(vox, nx), (unsigned long long *)xb, (unsigned long long *)yb,
                        (unsigned long long *)zb, (unsigned long long *)rb);
     MXB_FHIP(hipGetLastError());
