@@ -163,7 +163,9 @@ static int run_steps_f32(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t 
         MXB_HIP(timed(MXB_K_ZPASS, s, [&] { return launch_line_pass_f32(cur, gz, ez, k, R, arith, s); }));
         const Epilogue ey{nullptr, cur, nullptr};
         MXB_HIP(timed(MXB_K_YPASS, s, [&] { return launch_line_pass_f32(tmp, gy, ey, k, R, arith, s); }));
-        float *dst = (last && out_f32 != nullptr) ? out_f32 : tmp;
+        // the last x-pass writes the float state only where someone reads it
+        const bool state_unread = last && out_f32 == nullptr && out_u16 != nullptr && final_state == nullptr;
+        float *dst = (last && out_f32 != nullptr) ? out_f32 : (state_unread ? nullptr : tmp);
         const Epilogue ex{d_mask, dst, last ? out_u16 : nullptr};
         MXB_HIP(timed(MXB_K_XPASS, s, [&] { return launch_line_pass_f32(cur, gx, ex, k, R, arith, s); }));
         if (last && out_f32 != nullptr) {
