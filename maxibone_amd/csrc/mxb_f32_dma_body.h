@@ -34,6 +34,9 @@ struct F32DmaShape {
 
 constexpr int kF32Slot = 16 * 64;        // floats per slot
 constexpr int kF32StagePitchV = 36;      // floats per line row (position-contiguous epilogue, 2 tiles)
+// floats per position row of the line-contiguous epilogue: 68 puts the four
+// row groups one MFMA result writes (4 kg + r) in distinct 16-bank groups
+constexpr int kF32StagePitchZ = 68;
 
 // Shared-memory layout of one block of the direct-load pass (bytes):
 // [ring | mask ring | stage | tap table].
@@ -42,7 +45,7 @@ struct F32DmaSmem {
     using S = F32DmaShape<RT, D>;
     static constexpr int RING = S::NSL * kF32Slot * 4;
     static constexpr int MRING = MASK ? S::NM * 1024 : 0;
-    static constexpr int STAGE = (ZSTAGE ? 16 * 64 : 64 * kF32StagePitchV) * 4;
+    static constexpr int STAGE = (ZSTAGE ? 16 * kF32StagePitchZ : 64 * kF32StagePitchV) * 4;
     static constexpr int TAPS = (4 * S::C + 32) * 4;
     static constexpr int BYTES = RING + MRING + STAGE + TAPS;
 };
@@ -157,9 +160,9 @@ __device__ __forceinline__ void f32_dma_body(const float *__restrict__ in, const
         const bool full = tpos + 16 <= a1;  // uniform; only a segment's last tile is partial
         if (ZSTAGE) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) stage[(4 * kg + r) * 64 + 16 * wv + col] = acc[r];
+            for (int r = 0; r < 4; ++r) stage[(4 * kg + r) * kF32StagePitchZ + 16 * wv + col] = acc[r];
             __syncthreads();
-            const v4f v = *reinterpret_cast<const v4f *>(&stage[zr * 64 + 4 * zq]);
+            const v4f v = *reinterpret_cast<const v4f *>(&stage[zr * kF32StagePitchZ + 4 * zq]);
             const int64_t p = tpos + zr;
             float *op = e.out_f32 + outer * g.out_outer + p * g.out_ax + line0 + 4 * zq;
             if (full) {

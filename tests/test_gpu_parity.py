@@ -339,3 +339,41 @@ def test_kernel_forms_agree(gpu, pass_kernel, prec):
                 assert np.array_equal(o, outs["ring"]), f"form {f or 'direct'} differs"
             for o in again:
                 assert np.array_equal(o, outs[""])
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_kernel_forms_agree_random(gpu, pass_kernel, prec):
+    """Seeded random shapes and radii (1..64): the default (direct-load) form
+    equals the register-ring form bit for bit, twice in a row."""
+    rng = np.random.default_rng(2026 if prec == "fp32" else 2027)
+    for case in range(10):
+        shape = tuple(int(v) for v in rng.integers(1, 90, 3))
+        if case % 3 == 0:  # aligned line counts exercise the direct-load forms
+            shape = (shape[0], 8 * max(1, shape[1] // 8), 16 * max(1, shape[2] // 16))
+        sigma = float(rng.uniform(0.3, 16.0))
+        reps = int(rng.integers(1, 5))
+        m = (rng.random(shape) > rng.uniform(0.8, 0.99)).astype(np.uint8)
+        k = gauss_kernel(sigma).astype(np.float32)
+        if k.size // 2 > 64:
+            k = gauss_kernel(15.0).astype(np.float32)
+        pass_kernel("ring")
+        want = gpu.diffusion(m, k, reps, precision=prec, u16=True, f32=True)
+        pass_kernel(None)
+        got = [gpu.diffusion(m, k, reps, precision=prec, u16=True, f32=True) for _ in range(2)]
+        _say("random forms", prec, shape, round(sigma, 2), reps)
+        for q, u in got:
+            assert np.array_equal(u, want[1]) and np.array_equal(q, want[0])
+
+
+def test_batched_random_shapes_equal_single_volume(gpu):
+    """Batched plans over random shapes and radii equal the single-volume path."""
+    rng = np.random.default_rng(31)
+    masks, kernels = [], []
+    for _ in range(7):
+        shape = tuple(int(v) for v in rng.integers(3, 70, 3))
+        masks.append((rng.random(shape) > 0.95).astype(np.uint8))
+        kernels.append(gauss_kernel(float(rng.uniform(0.5, 12.0))).astype(np.float32))
+    qs, fs = gpu.diffusion_batched(masks, kernels, 3, precision="fp32", u16=True, f32=True)
+    for m, k, q, f in zip(masks, kernels, qs, fs):
+        q1, f1 = gpu.diffusion(m, k, 3, precision="fp32", u16=True, f32=True)
+        assert np.array_equal(f, f1) and np.array_equal(q, q1)
