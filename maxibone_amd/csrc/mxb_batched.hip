@@ -55,7 +55,7 @@ struct BatchItem {
 template <int FORM, int RT>
 __device__ __forceinline__ void batch_body(const BatchPassDesc &d, const BatchItem &it,
                                            const float *__restrict__ taps, unsigned char *smem) {
-    f32_dma_body<RT, MXB_F32_DMA_DEPTH, FORM == 2, true, false, FORM == 0>(
+    f32_dma_pass<RT, MXB_F32_DMA_DEPTH, FORM == 2, true, false, FORM == 0>(
         d.in, d.g, d.e, it.outer, it.lb, it.seg, taps + d.taps_off, smem);
 }
 
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void k_batched_pass(const BatchPassDesc *__res
 
 template <int FORM, int RT>
 static constexpr int form_smem() {
-    return F32DmaSmem<RT, MXB_F32_DMA_DEPTH, FORM == 2, FORM == 0>::BYTES;
+    return F32DmaForm<RT, MXB_F32_DMA_DEPTH, FORM == 2, FORM == 0>::BYTES;
 }
 
 template <int FORM>

@@ -21,14 +21,14 @@ __global__ __launch_bounds__(256) void k_line_mfma_lds(const float *__restrict__
 template <int RT, int D, bool MASK, bool OUTF, bool OUTU, bool ZSTAGE>
 __global__ __launch_bounds__(256) void k_line_f32_dma(const float *__restrict__ in, LineGeom g,
                                                       Epilogue e, TapsF<2 * RT + 1> taps) {
-    __shared__ __attribute__((aligned(16))) unsigned char smem[F32DmaSmem<RT, D, MASK, ZSTAGE>::BYTES];
+    __shared__ __attribute__((aligned(16))) unsigned char smem[F32DmaForm<RT, D, MASK, ZSTAGE>::BYTES];
     int64_t bid = blockIdx.x;
     const int64_t seg_id = bid % g.nseg;
     bid /= g.nseg;
     const int64_t nlb = (g.n_lane + 63) >> 6;
     const int64_t lb = bid % nlb;
     const int64_t outer = bid / nlb;
-    f32_dma_body<RT, D, MASK, OUTF, OUTU, ZSTAGE>(in, g, e, outer, lb, seg_id, taps.k, smem);
+    f32_dma_pass<RT, D, MASK, OUTF, OUTU, ZSTAGE>(in, g, e, outer, lb, seg_id, taps.k, smem);
 }
 
 // Which direct-load epilogue serves (g, e): 0 none (use the VGPR-staged
