@@ -32,7 +32,10 @@ extern "C" {
 int mxb_combine_fields(const uint16_t *gauss, const uint16_t *edt, int64_t nz, int64_t ny,
                        int64_t nx, const uint8_t *cylinder, uint16_t *out, int verbose);
 
-/* Device form: bytes of the workspace it needs (a few KB). */
+/* Device form: bytes of the workspace it needs (a few KB).  The passes are
+ * enqueued on `stream`, and the call synchronises it twice (the host derives
+ * each pass's parameters from the previous reductions); it returns with the
+ * output written. */
 int64_t mxb_combine_workspace_bytes(void);
 int mxb_device_combine_fields(const uint16_t *dev_gauss, const uint16_t *dev_edt, int64_t nz,
                               int64_t ny, int64_t nx, const uint8_t *dev_cylinder,
@@ -62,7 +65,8 @@ int mxb_axis_histograms(const uint16_t *voxels, int64_t nz, int64_t ny, int64_t 
                         uint64_t cx, double vmin, double vmax, int verbose);
 
 /* Device forms of the histograms: all pointers in HBM, enqueued on `stream`
- * (the bins are added to).  The workspace holds the bin lookup tables and
+ * (the bins are added to); each call returns after synchronising `stream`
+ * (its lookup tables are staged from host memory).  The workspace holds the bin lookup tables and
  * coordinate maps; mxb_histogram_workspace_bytes gives its size for the
  * larger of the two uses with these extents. */
 int64_t mxb_histogram_workspace_bytes(int64_t nZ, int64_t nY, int64_t nX);

@@ -570,8 +570,10 @@ int mxb_device_combine_fields(const uint16_t *dev_gauss, const uint16_t *dev_edt
     }
     std::vector<uint8_t> cyl;
     if (dev_cylinder != nullptr && ny * nx > 0) {
+        // ordered after the caller's work on its stream (which may produce the mask)
         cyl.resize((size_t)(ny * nx));
-        MXB_FHIP(hipMemcpy(cyl.data(), dev_cylinder, cyl.size(), hipMemcpyDeviceToHost));
+        MXB_FHIP(hipMemcpyAsync(cyl.data(), dev_cylinder, cyl.size(), hipMemcpyDeviceToHost, (hipStream_t)stream));
+        MXB_FHIP(hipStreamSynchronize((hipStream_t)stream));
     }
     return device_combine(dev_gauss, dev_edt, nz, ny, nx, dev_cylinder,
                           dev_cylinder != nullptr ? cyl.data() : nullptr, dev_out, (uint32_t *)dev_workspace,
