@@ -135,7 +135,9 @@ int mxb_batch_destroy(mxb_batch_plan *plan);
 
 /* ---- device-resident form (inputs already in HBM) -------------------------- */
 
-/* Bytes of one state buffer for `precision` (two are needed); -1 (and
+/* Bytes of one state buffer for `precision` (two are needed): the FP32 state
+ * rounded up to 256 bytes, then room for the mask packed to bits (one 32-bit
+ * word per 32 voxels of an x row), which the first buffer carries.  -1 (and
  * mxb_last_error set) for an unknown precision or a negative extent. */
 int64_t mxb_state_bytes(int precision, int64_t nz, int64_t ny, int64_t nx);
 

@@ -149,6 +149,7 @@ struct VolLayout {
     int64_t nz, ny, nx, px, py;
     int64_t st_off;    // floats, into each state buffer
     int64_t mask_off;  // bytes, into the pitched masks
+    int64_t bits_off;  // words, into the pitched mask bits ((nz, py) rows of words_per_row(px))
     int R, bucket;
 };
 
@@ -197,6 +198,7 @@ struct mxb_batch_plan {
     std::vector<std::vector<float>> host_taps;
     void *ws = nullptr;  // one device workspace
     uint8_t *masks = nullptr;
+    uint32_t *bits = nullptr;
     float *st[2] = {nullptr, nullptr};
     float *taps = nullptr;
     mxb::BatchPassDesc *descs = nullptr;  // [6][n]: (parity, pass)
@@ -269,7 +271,10 @@ static int build_tables(mxb_batch_plan *P, int seg) {
                 D.in = ins[p];
                 D.g = pitched_geom(L, p);
                 set_segments(D.g, seg);
-                D.e = Epilogue{p == 2 ? P->masks + L.mask_off : nullptr, outs[p], nullptr};
+                const int64_t wpr = mask_words_per_row(L.px);
+                D.e = p == 2 ? Epilogue{P->masks + L.mask_off, outs[p], nullptr,
+                                        MaskBits{P->bits + L.bits_off, wpr, L.py * wpr}}
+                             : Epilogue{nullptr, outs[p], nullptr, MaskBits{}};
                 D.bucket = L.bucket;
                 D.taps_off = 0;
             }
@@ -476,7 +481,7 @@ int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, 
     P->vols.assign(vols, vols + n);
     // the persistent form serves FP32 (MFMA) volumes with R <= 64
     P->persistent = precision == MXB_FP32;
-    int64_t st_total = 0, mask_total = 0;
+    int64_t st_total = 0, mask_total = 0, bits_total = 0;
     std::string key = "b" + std::to_string(n) + "p" + std::to_string(precision);
     for (int v = 0; v < n; ++v) {
         const mxb_device_volume &V = vols[v];
@@ -496,9 +501,13 @@ int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, 
         for (int b = 0; b < kNumBuckets; ++b)
             if (L.R <= kBucketRT[b]) { L.bucket = b; break; }
         L.st_off = st_total;
-        st_total += round_up(V.nz * L.py * L.px, 64);
+        // the per-volume form runs mxb_device_diffusion on these buffers
+        const int64_t sb = mxb_state_bytes(precision, V.nz, V.ny, V.nx);
+        st_total += round_up(std::max(V.nz * L.py * L.px, (sb + 3) / 4), 64);
         L.mask_off = mask_total;
         mask_total += round_up(V.nz * L.py * L.px, 256);
+        L.bits_off = bits_total;
+        bits_total += round_up(mask_bits_bytes(V.nz, L.py, L.px) / 4, 64);
         P->lay.push_back(L);
         P->host_taps.emplace_back(V.kernel, V.kernel + V.kernel_size);
         key += "_" + std::to_string(V.nz) + "x" + std::to_string(V.ny) + "x" + std::to_string(V.nx) + "r" +
@@ -525,6 +534,7 @@ int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, 
         return o;
     };
     const size_t o_mask = carve((size_t)mask_total);
+    const size_t o_bits = carve((size_t)bits_total * 4);
     const size_t o_s0 = carve((size_t)st_total * 4);
     const size_t o_s1 = carve((size_t)st_total * 4);
     const size_t o_taps = carve((size_t)taps_cap * 4);
@@ -538,6 +548,7 @@ int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, 
     }
     char *base = (char *)P->ws;
     P->masks = (uint8_t *)(base + o_mask);
+    P->bits = (uint32_t *)(base + o_bits);
     P->st[0] = (float *)(base + o_s0);
     P->st[1] = (float *)(base + o_s1);
     P->taps = (float *)(base + o_taps);
@@ -559,6 +570,11 @@ int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, 
                            vols[v].dev_mask, L.nz, L.ny, L.nx, L.py, L.px, P->masks + L.mask_off);
         if ((e = hipGetLastError()) != hipSuccess) {
             set_error(std::string("k_pitch_mask: ") + hipGetErrorString(e));
+            return fail(EDEVICE);
+        }
+        if ((e = launch_pack_mask_bits(P->masks + L.mask_off, L.nz * L.py, L.px, L.px, P->bits + L.bits_off,
+                                       mask_words_per_row(L.px), P->stream)) != hipSuccess) {
+            set_error(std::string("k_pack_mask_bits: ") + hipGetErrorString(e));
             return fail(EDEVICE);
         }
     }

@@ -128,10 +128,22 @@ static bool is_lowp(int precision) { return precision == MXB_BF16 || precision =
 
 // Every form needs two FP32-sized state buffers: the 16-bit forms use the
 // first half of each for their repetitions and the whole of both for the
-// final FP32 repetition.
+// final FP32 repetition.  The first buffer also carries the mask packed to
+// bits after the state (at byte state_f32_bytes).
+static int64_t state_f32_bytes(int64_t n) { return (n * 4 + 255) / 256 * 256; }
+
 int64_t state_bytes(int precision, int64_t nz, int64_t ny, int64_t nx) {
     (void)precision;
-    return nz * ny * nx * 4;
+    return state_f32_bytes(nz * ny * nx) + mask_bits_bytes(nz, ny, nx);
+}
+
+// Packs the (nz, ny, nx) device mask to bits at `w` and describes them for
+// the x-pass (lines y, outer index z).
+static hipError_t pack_bits(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t nx, uint32_t *w,
+                            MaskBits &bits, hipStream_t s) {
+    const int64_t wpr = mask_words_per_row(nx);
+    bits = MaskBits{w, wpr, ny * wpr};
+    return launch_pack_mask_bits(d_mask, nz * ny, nx, nx, w, wpr, s);
 }
 
 // ---------------------------------------------------------------- driver
@@ -141,8 +153,8 @@ int64_t state_bytes(int precision, int64_t nz, int64_t ny, int64_t nx) {
 // illumination; then the buffers swap roles.  The final x-pass writes
 // out_f32 / out_u16 (when non-null); when out_f32 is null the final FP32
 // state is left in the buffer returned through *final_state (may be null).
-static int run_steps_f32(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t nx,
-                         const float *k, int64_t K, int64_t reps, Arith arith, float *s0,
+static int run_steps_f32(const uint8_t *d_mask, const MaskBits &bits, int64_t nz, int64_t ny,
+                         int64_t nx, const float *k, int64_t K, int64_t reps, Arith arith, float *s0,
                          float *s1, bool init_from_mask, uint16_t *out_u16, float *out_f32,
                          float **final_state, hipStream_t s) {
     const int64_t n = nz * ny * nx;
@@ -159,14 +171,14 @@ static int run_steps_f32(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t 
     const LineGeom gz = geom_z(nz, ny, nx), gy = geom_y(nz, ny, nx), gx = geom_x(nz, ny, nx);
     for (int64_t t = 0; t < reps; ++t) {
         const bool last = t == reps - 1;
-        const Epilogue ez{nullptr, tmp, nullptr};
+        const Epilogue ez{nullptr, tmp, nullptr, MaskBits{}};
         MXB_HIP(timed(MXB_K_ZPASS, s, [&] { return launch_line_pass_f32(cur, gz, ez, k, R, arith, s); }));
-        const Epilogue ey{nullptr, cur, nullptr};
+        const Epilogue ey{nullptr, cur, nullptr, MaskBits{}};
         MXB_HIP(timed(MXB_K_YPASS, s, [&] { return launch_line_pass_f32(tmp, gy, ey, k, R, arith, s); }));
         // the last x-pass writes the float state only where someone reads it
         const bool state_unread = last && out_f32 == nullptr && out_u16 != nullptr && final_state == nullptr;
         float *dst = (last && out_f32 != nullptr) ? out_f32 : (state_unread ? nullptr : tmp);
-        const Epilogue ex{d_mask, dst, last ? out_u16 : nullptr};
+        const Epilogue ex{d_mask, dst, last ? out_u16 : nullptr, bits};
         MXB_HIP(timed(MXB_K_XPASS, s, [&] { return launch_line_pass_f32(cur, gx, ex, k, R, arith, s); }));
         if (last && out_f32 != nullptr) {
             if (final_state) *final_state = out_f32;
@@ -182,44 +194,49 @@ static int run_steps_f32(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t 
 // y: s1 -> s0, x: s0 -> s1 + illumination, swap); the state is then widened
 // to FP32 and the last repetition runs in FP32 (MFMA), so the result carries
 // no rounding of its own repetition.
-static int run_steps_lowp(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t nx,
-                          const float *k, int64_t K, int64_t reps, bool bf, void *s0, void *s1,
-                          uint16_t *out_u16, float *out_f32, hipStream_t s) {
+static int run_steps_lowp(const uint8_t *d_mask, const MaskBits &bits, int64_t nz, int64_t ny,
+                          int64_t nx, const float *k, int64_t K, int64_t reps, bool bf, void *s0,
+                          void *s1, uint16_t *out_u16, float *out_f32, hipStream_t s) {
     const int64_t n = nz * ny * nx;
     const int R = (int)(K / 2);
     if (n == 0) return OK;
     if (reps <= 1)
-        return run_steps_f32(d_mask, nz, ny, nx, k, K, reps, Arith::MFMA, (float *)s0, (float *)s1,
-                             true, out_u16, out_f32, nullptr, s);
+        return run_steps_f32(d_mask, bits, nz, ny, nx, k, K, reps, Arith::MFMA, (float *)s0,
+                             (float *)s1, true, out_u16, out_f32, nullptr, s);
     uint16_t *cur = (uint16_t *)s0, *tmp = (uint16_t *)s1;
     MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_init_lowp(d_mask, cur, n, bf, s); }));
     const LineGeom gz = geom_z(nz, ny, nx), gy = geom_y(nz, ny, nx), gx = geom_x(nz, ny, nx);
     for (int64_t t = 0; t < reps - 1; ++t) {
         MXB_HIP(timed(MXB_K_ZPASS, s, [&] { return launch_line_pass_lowp(cur, tmp, nullptr, gz, k, R, bf, s); }));
         MXB_HIP(timed(MXB_K_YPASS, s, [&] { return launch_line_pass_lowp(tmp, cur, nullptr, gy, k, R, bf, s); }));
-        MXB_HIP(timed(MXB_K_XPASS, s, [&] { return launch_line_pass_lowp(cur, tmp, d_mask, gx, k, R, bf, s); }));
+        MXB_HIP(timed(MXB_K_XPASS, s, [&] { return launch_line_pass_lowp(cur, tmp, d_mask, gx, k, R, bf, s, bits); }));
         std::swap(cur, tmp);
     }
     // widen into the other buffer, then one FP32 repetition
     float *f0 = (float *)(cur == (uint16_t *)s0 ? s1 : s0);
     float *f1 = (float *)(cur == (uint16_t *)s0 ? s0 : s1);
     MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_lowp_to_f32(cur, f0, n, bf, s); }));
-    return run_steps_f32(d_mask, nz, ny, nx, k, K, 1, Arith::MFMA, f0, f1, false, out_u16, out_f32,
-                         nullptr, s);
+    return run_steps_f32(d_mask, bits, nz, ny, nx, k, K, 1, Arith::MFMA, f0, f1, false, out_u16,
+                         out_f32, nullptr, s);
 }
 
 static int run_steps(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t nx, const float *k,
                      int64_t K, int64_t reps, int precision, void *s0, void *s1,
                      uint16_t *out_u16, float *out_f32, hipStream_t s) {
+    MaskBits bits;
+    if (reps > 0 && nz * ny * nx > 0) {
+        uint32_t *w = reinterpret_cast<uint32_t *>(static_cast<char *>(s0) + state_f32_bytes(nz * ny * nx));
+        MXB_HIP(timed(MXB_K_OTHER, s, [&] { return pack_bits(d_mask, nz, ny, nx, w, bits, s); }));
+    }
     if (is_lowp(precision)) {
         if (K / 2 > kMaxRadiusLowp) {
             set_error("BF16/FP16 forms support kernel radius <= " + std::to_string(kMaxRadiusLowp));
             return EINVAL_;
         }
-        return run_steps_lowp(d_mask, nz, ny, nx, k, K, reps, precision == MXB_BF16, s0, s1,
+        return run_steps_lowp(d_mask, bits, nz, ny, nx, k, K, reps, precision == MXB_BF16, s0, s1,
                               out_u16, out_f32, s);
     }
-    return run_steps_f32(d_mask, nz, ny, nx, k, K, reps, arith_of(precision), (float *)s0,
+    return run_steps_f32(d_mask, bits, nz, ny, nx, k, K, reps, arith_of(precision), (float *)s0,
                          (float *)s1, true, out_u16, out_f32, nullptr, s);
 }
 
@@ -267,8 +284,9 @@ static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
     const int64_t cap = max_layers * layer;
     Stream st;
     MXB_HIP(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
-    DevBuf dm, d0, d1, dq;
+    DevBuf dm, d0, d1, dq, db;
     MXB_HIP(dm.alloc(cap));
+    MXB_HIP(db.alloc(mask_bits_bytes(max_layers, ny, nx)));
     MXB_HIP(d0.alloc(cap * 4));
     MXB_HIP(d1.alloc(cap * 4));
     if (out_u16) MXB_HIP(dq.alloc(cap * 2));
@@ -299,8 +317,10 @@ static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
             log_v(verbose, 2, "chunk z0=" + std::to_string(z0) + " layers=" + std::to_string(lz));
             MXB_HIP(hipMemcpyAsync(dm.p, mask + lo * layer, lz * layer, hipMemcpyHostToDevice, st.s));
             MXB_HIP(hipMemcpyAsync(d0.p, cur + lo * layer, lz * layer * 4, hipMemcpyHostToDevice, st.s));
+            MaskBits bits;
+            MXB_HIP(pack_bits((const uint8_t *)dm.p, lz, ny, nx, (uint32_t *)db.p, bits, st.s));
             float *fin = nullptr;
-            int rc = run_steps_f32((const uint8_t *)dm.p, lz, ny, nx, k, K, tb, arith, (float *)d0.p,
+            int rc = run_steps_f32((const uint8_t *)dm.p, bits, lz, ny, nx, k, K, tb, arith, (float *)d0.p,
                                    (float *)d1.p, false, last_block && out_u16 ? (uint16_t *)dq.p : nullptr,
                                    nullptr, &fin, st.s);
             if (rc != OK) return rc;

@@ -80,6 +80,18 @@ __device__ __forceinline__ void dma16(uint32_t lds_wave_base, const void *src) {
 #pragma clang diagnostic pop
 }
 
+// 4 bytes per active lane: lane i writes LDS bytes [base + 4 i, +4) from the
+// 4-byte-aligned global address `src` (same wait-count rules as dma16).
+__device__ __forceinline__ void dma4(uint32_t lds_wave_base, const void *src) {
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dword %1, off"
+                 :
+                 : "s"(lds_wave_base), "v"(src)
+                 : "memory", "m0");
+#pragma clang diagnostic pop
+}
+
 // s_waitcnt with vmcnt = N and no wait on the other counters.
 template <int N>
 __device__ __forceinline__ void vm_wait() {

@@ -67,6 +67,7 @@ __device__ __forceinline__ void f32_dma_body(const float *__restrict__ in, const
     // (position-contiguous: a pair of tiles makes 128-byte line pieces)
     constexpr int ST = ZSTAGE ? 1 : (OUTF ? 2 : 0) + (OUTU ? 1 : 0);
     static_assert(ZSTAGE || D % 2 == 0, "pair flushes keep the store count fixed only for even D");
+    static_assert(!MASK, "the illuminating pass runs on f32_dma_body2 (mask bits)");
     static_assert(!ZSTAGE || (OUTF && !OUTU && !MASK), "line-contiguous epilogue writes floats only");
     static_assert(ST >= 1, "at least one output");
     using M = F32DmaSmem<RT, D, MASK, ZSTAGE>;
@@ -254,7 +255,7 @@ template <int RT, int D, bool MASK, bool ZSTAGE>
 struct F32Dma2Smem {
     using S = F32Dma2Shape<RT, D>;
     static constexpr int RING = S::NSL * kF32Slot * 4;
-    static constexpr int MRING = MASK ? S::NM * 2048 : 0;
+    static constexpr int MRING = MASK ? S::NM * 256 : 0;  // 64 lines x one 32-bit mask word
     static constexpr int STAGE = (ZSTAGE ? 32 * kF32StagePitchZ : 64 * kF32StagePitchV) * 4;
     static constexpr int TAPS = (4 * S::C + 32) * 4;
     static constexpr int BYTES = RING + MRING + STAGE + TAPS;
@@ -309,17 +310,17 @@ __device__ __forceinline__ void f32_dma_body2(const float *__restrict__ in, cons
                              : static_cast<const void *>(g_dma_zero);
         dma16<ZSTAGE>(ring_lds + (uint32_t)((slot % NSL) * kF32Slot * 4), src);
     };
-    // ---- mask loads (lanes < 32): line 16 wv + lane / 2, 16 positions each
-    const int ml = 16 * wv + (lane >> 1);
+    // ---- mask bits (lanes < 16): the 32-bit word of the pair's 32 positions of
+    // line 16 wv + lane (segments start at multiples of 32 positions)
+    const int ml = 16 * wv + lane;
     const bool ml_ok = MASK && line0 + ml < g.n_lane;
-    const uint8_t *msrc = MASK ? e.mask + outer * g.out_outer + (line0 + ml) * g.out_lane + 16 * (lane & 1)
-                               : nullptr;
-    const uint32_t mring_lds = lds_addr(mring) + 512u * (uint32_t)wv;
+    const uint32_t *msrc = MASK ? e.bits.w + outer * e.bits.outer + (line0 + ml) * e.bits.lane : nullptr;
+    const uint32_t mring_lds = lds_addr(mring) + 64u * (uint32_t)wv;
     auto issue_mask = [&](int pj) {
         const int64_t ppos = a0 + 32 * (int64_t)pj;
         const bool ok = ml_ok && ppos + 32 <= a1;
-        const void *src = ok ? static_cast<const void *>(msrc + ppos) : static_cast<const void *>(g_dma_zero);
-        if (lane < 32) dma16<false>(mring_lds + (uint32_t)((pj % NM) * 2048), src);
+        const void *src = ok ? static_cast<const void *>(msrc + ppos / 32) : static_cast<const void *>(g_dma_zero);
+        if (lane < 16) dma4(mring_lds + (uint32_t)((pj % NM) * 256), src);
     };
     auto issue_group = [&](int i) {
         issue_data(2 * i + W - 1);
@@ -397,11 +398,11 @@ __device__ __forceinline__ void f32_dma_body2(const float *__restrict__ in, cons
             const int64_t off = outer * g.out_outer + (line0 + sl) * g.out_lane + p0;
             if (full) {
                 if (MASK) {
-                    const uint2 m = *reinterpret_cast<const uint2 *>(mring + (j % NM) * 2048 + sl * 32 + 8 * sq);
+                    const uint32_t m = *reinterpret_cast<const uint32_t *>(mring + (j % NM) * 256 + sl * 4) >> (8 * sq);
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        if ((m.x >> (8 * r)) & 0xffu) v[0][r] = 1.0f;
-                        if ((m.y >> (8 * r)) & 0xffu) v[1][r] = 1.0f;
+                        if ((m >> r) & 1u) v[0][r] = 1.0f;
+                        if ((m >> (4 + r)) & 1u) v[1][r] = 1.0f;
                     }
                 }
                 if (OUTF) {

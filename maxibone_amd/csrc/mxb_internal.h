@@ -33,12 +33,28 @@ struct LineGeom {
     int32_t stage_out;               // 1: out_lane == 1, rows 16-byte aligned: stage tiles in LDS
 };
 
+// The mask packed to bits along x: bit (x mod 32) of word
+// [(z ny + y) * words_per_row(nx) + x / 32] is (mask[z][y][x] != 0).  The
+// illuminating pass's direct-load forms read it (1/8 byte per voxel instead of
+// 1 byte); every other form reads the byte mask.
+struct MaskBits {
+    const uint32_t *w = nullptr;  // nullptr: not available
+    int64_t lane = 0, outer = 0;  // word strides of the pass's lines and outer index
+};
+inline int64_t mask_words_per_row(int64_t nx) { return (nx + 31) / 32; }
+inline int64_t mask_bits_bytes(int64_t nz, int64_t ny, int64_t nx) {
+    return nz * ny * mask_words_per_row(nx) * 4;
+}
+hipError_t launch_pack_mask_bits(const uint8_t *mask, int64_t rows, int64_t nx, int64_t mask_row,
+                                 uint32_t *bits, int64_t bits_row, hipStream_t s);
+
 // Epilogue of the last pass of a step: illuminate from the mask, write the
 // float state and/or its uint16 quantisation.
 struct Epilogue {
     const uint8_t *mask;   // nullptr: plain pass (no illumination)
     float *out_f32;        // may be nullptr when only out_u16 is wanted
     uint16_t *out_u16;     // nullptr unless this is the final step
+    MaskBits bits;         // the same mask as bits, when available
 };
 
 enum class Arith : int { FMA = 0, EXACT = 1, MFMA = 2 };
@@ -73,7 +89,7 @@ hipError_t launch_line_pass_f32_lds(const float *in, const LineGeom &g, const Ep
 // `bf` selects BF16 (else FP16).  `mask` non-null illuminates.
 hipError_t launch_line_pass_lowp(const uint16_t *in, uint16_t *out, const uint8_t *mask,
                                  const LineGeom &g, const float *taps, int R, bool bf,
-                                 hipStream_t s);
+                                 hipStream_t s, MaskBits bits = MaskBits{});
 hipError_t launch_init_lowp(const uint8_t *mask, uint16_t *u, int64_t n, bool bf, hipStream_t s);
 hipError_t launch_lowp_to_f32(const uint16_t *u, float *f, int64_t n, bool bf, hipStream_t s);
 constexpr int kMaxRadiusLowp = 64;

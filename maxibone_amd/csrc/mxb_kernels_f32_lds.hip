@@ -40,8 +40,7 @@ static int f32_dma_form(const LineGeom &g, const Epilogue &e) {
     if (u && reinterpret_cast<uintptr_t>(e.out_u16) % 16 != 0) return 0;
     if (g.out_lane == 1 && !m && !u && f && g.out_outer % 4 == 0 && g.out_ax % 4 == 0) return 1;
     if (g.out_ax != 1 || g.out_lane % 8 != 0 || g.out_outer % 8 != 0) return 0;
-    if (m && (g.out_lane % 16 != 0 || g.out_outer % 16 != 0 || reinterpret_cast<uintptr_t>(e.mask) % 16 != 0))
-        return 0;
+    if (m && e.bits.w == nullptr) return 0;  // the direct-load illuminating pass reads mask bits
     return 2 + (m ? 4 : 0) + (f ? 1 : 0) + (u ? 2 : 0);
 }
 
@@ -84,7 +83,8 @@ static hipError_t launch_lds_rt(const float *in, LineGeom g, const Epilogue &e, 
     if (nseg > max_seg) nseg = max_seg;
     if (nseg < 1) nseg = 1;
     int64_t seg = (g.n_ax + nseg - 1) / nseg;
-    seg = ((seg + 15) / 16) * 16;
+    const int64_t q = e.mask != nullptr ? 32 : 16;  // the illuminating pass: whole mask-bit words
+    seg = ((seg + q - 1) / q) * q;
     g.seg = seg;
     g.nseg = (g.n_ax + seg - 1) / seg;
     g.vec_out = (g.out_ax == 1 && g.out_lane % 4 == 0 && g.out_outer % 4 == 0 &&

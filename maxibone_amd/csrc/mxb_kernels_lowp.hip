@@ -203,9 +203,9 @@ static bool lowp_offsets_fit(const LineGeom &g) {
 
 hipError_t launch_line_pass_lowp(const uint16_t *in, uint16_t *out, const uint8_t *mask,
                                  const LineGeom &g, const float *taps, int R, bool bf,
-                                 hipStream_t s) {
+                                 hipStream_t s, MaskBits bits) {
     if (!force_ring_kernels() && lowp_lds_supported(in, g, R))
-        return launch_line_pass_lowp_lds(in, out, mask, g, taps, R, bf, s);
+        return launch_line_pass_lowp_lds(in, out, mask, bits, g, taps, R, bf, s);
     if (!lowp_offsets_fit(g)) return hipErrorInvalidValue;
 #define MXB_LOWP_CASE(rt) \
     if (R <= rt) return launch_lowp_rt<rt>(in, out, mask, g, taps, R, bf, s);

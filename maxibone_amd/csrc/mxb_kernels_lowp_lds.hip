@@ -248,8 +248,9 @@ __global__ __launch_bounds__(256) void k_line_lowp_lds(const uint16_t *__restric
 
 // ---------------------------------------------------------------------------
 // Direct-load form: the same pass with the slots (and, for the illuminating
-// pass, 64-line x 32-position mask tiles) moved global -> LDS by
-// global_load_lds_dwordx4, D slots ahead of use, with no VGPR staging.
+// pass, the 64 lines' 32-bit mask words of each 32-position pair) moved
+// global -> LDS by global_load_lds_dword{x4,}, D slots ahead of use, with no
+// VGPR staging.
 //
 // Iteration j (tile pair j):
 //   A. wait until group j has landed (vm_wait), barrier;
@@ -273,13 +274,14 @@ struct LowpDmaShape {
     static constexpr int NM = D + 1;   // mask tile slots
 };
 
-constexpr int kMaskSlotBytes = 64 * 32;  // 64 lines x 32 positions
+constexpr int kMaskSlotBytes = 64 * 4;  // 64 lines x one 32-bit word of mask bits
 
 template <int RT, int D, bool BF, bool MASK, bool ZSTAGE>
 __global__ __launch_bounds__(256) void k_line_lowp_dma(const uint16_t *__restrict__ in,
                                                        uint16_t *__restrict__ out,
                                                        const uint8_t *__restrict__ mask,
-                                                       LineGeom g, TapsL<2 * RT + 1> taps) {
+                                                       MaskBits bits, LineGeom g,
+                                                       TapsL<2 * RT + 1> taps) {
     using S = LowpDmaShape<RT, D>;
     using L = Lowp<BF>;
     constexpr int C = S::C, W = S::W, NSL = S::NSL, NM = S::NM;
@@ -327,17 +329,17 @@ __global__ __launch_bounds__(256) void k_line_lowp_dma(const uint16_t *__restric
                              : static_cast<const void *>(g_dma_zero);
         dma16<ZSTAGE>(ring_lds + (uint32_t)((slot % NSL) * kSlotBytes), src);
     };
-    // ---- mask loads (lanes < 32): line 16 wv + lane / 2, 16 positions each
-    const int ml = 16 * wv + (lane >> 1);
+    // ---- mask bits (lanes < 16): the 32-bit word of the pair's 32 positions of
+    // line 16 wv + lane (segments start at multiples of 32 positions)
+    const int ml = 16 * wv + lane;
     const bool ml_ok = MASK && line0 + ml < g.n_lane;
-    const uint8_t *msrc = MASK ? mask + outer * g.out_outer + (line0 + ml) * g.out_lane + 16 * (lane & 1)
-                               : nullptr;
-    const uint32_t mring_lds = lds_addr(mring) + 512u * (uint32_t)wv;
+    const uint32_t *msrc = MASK ? bits.w + outer * bits.outer + (line0 + ml) * bits.lane : nullptr;
+    const uint32_t mring_lds = lds_addr(mring) + 64u * (uint32_t)wv;
     auto issue_mask = [&](int pj) {
         const int64_t ppos = a0 + 32 * (int64_t)pj;
         const bool ok = ml_ok && ppos + 32 <= a1;
-        const void *src = ok ? static_cast<const void *>(msrc + ppos) : static_cast<const void *>(g_dma_zero);
-        if (lane < 32) dma16<false>(mring_lds + (uint32_t)((pj % NM) * kMaskSlotBytes), src);
+        const void *src = ok ? static_cast<const void *>(msrc + ppos / 32) : static_cast<const void *>(g_dma_zero);
+        if (lane < 16) dma4(mring_lds + (uint32_t)((pj % NM) * kMaskSlotBytes), src);
     };
     auto issue_group = [&](int i) {
         issue_data(i + W - 1);
@@ -433,13 +435,13 @@ __global__ __launch_bounds__(256) void k_line_lowp_dma(const uint16_t *__restric
             const int64_t off = outer * g.out_outer + (line0 + sl) * g.out_lane + p0;
             if (full) {
                 if (MASK) {
-                    const uint2 m = *reinterpret_cast<const uint2 *>(mring + (j % NM) * kMaskSlotBytes + sl * 32 + 8 * sq);
+                    const uint32_t m =
+                        *reinterpret_cast<const uint32_t *>(mring + (j % NM) * kMaskSlotBytes + sl * 4) >> (8 * sq);
                     const uint32_t one = L::from_f32(1.0f);
                     uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
                     for (int e = 0; e < 8; ++e) {
-                        const uint32_t mb = ((e < 4 ? m.x : m.y) >> (8 * (e & 3))) & 0xffu;
-                        if (mb != 0) {
+                        if ((m >> e) & 1u) {
                             const int sh = 16 * (e & 1);
                             w[e >> 1] = (w[e >> 1] & ~(0xffffu << sh)) | (one << sh);
                         }
@@ -497,21 +499,21 @@ static hipError_t launch_lowp_lds_rt(const uint16_t *in, uint16_t *out, const ui
 
 // Direct-load form: needs whole 8-line row pieces (n_lane a multiple of 8) and
 // 16-byte-aligned output rows written either line-contiguous or
-// position-contiguous (mask rows 16-byte aligned too).
-static int lowp_dma_form(const uint16_t *out, const uint8_t *mask, const LineGeom &g) {
+// position-contiguous (the illuminating pass reads the mask as bits).
+static int lowp_dma_form(const uint16_t *out, const uint8_t *mask, const MaskBits &bits, const LineGeom &g) {
     if (g.n_lane % 8 != 0 || reinterpret_cast<uintptr_t>(out) % 16 != 0) return 0;
     if (mask == nullptr && g.out_lane == 1 && g.out_outer % 8 == 0 && g.out_ax % 8 == 0) return 1;
     if (g.out_ax == 1 && g.out_lane % 8 == 0 && g.out_outer % 8 == 0) {
         if (mask == nullptr) return 2;
-        if (g.out_lane % 16 == 0 && g.out_outer % 16 == 0 && reinterpret_cast<uintptr_t>(mask) % 16 == 0)
-            return 3;
+        if (bits.w != nullptr) return 3;
     }
     return 0;
 }
 
 template <int RT, bool BF>
 static hipError_t launch_lowp_dma_rt(int form, const uint16_t *in, uint16_t *out, const uint8_t *mask,
-                                     LineGeom g, const float *k, int R, hipStream_t s) {
+                                     const MaskBits &bits, LineGeom g, const float *k, int R,
+                                     hipStream_t s) {
     constexpr int D = MXB_LOWP_DMA_DEPTH;
     const int64_t groups = ((g.n_lane + 63) / 64) * g.n_outer;
     const int64_t target = 256LL * 6;
@@ -527,21 +529,22 @@ static hipError_t launch_lowp_dma_rt(int form, const uint16_t *in, uint16_t *out
     if (blocks == 0) return hipSuccess;
     const TapsL<2 * RT + 1> t = split_taps<RT>(k, R, BF);
     if (form == 1)
-        hipLaunchKernelGGL((k_line_lowp_dma<RT, D, BF, false, true>), dim3((unsigned)blocks), dim3(256), 0, s, in, out, mask, g, t);
+        hipLaunchKernelGGL((k_line_lowp_dma<RT, D, BF, false, true>), dim3((unsigned)blocks), dim3(256), 0, s, in, out, mask, bits, g, t);
     else if (form == 2)
-        hipLaunchKernelGGL((k_line_lowp_dma<RT, D, BF, false, false>), dim3((unsigned)blocks), dim3(256), 0, s, in, out, mask, g, t);
+        hipLaunchKernelGGL((k_line_lowp_dma<RT, D, BF, false, false>), dim3((unsigned)blocks), dim3(256), 0, s, in, out, mask, bits, g, t);
     else
-        hipLaunchKernelGGL((k_line_lowp_dma<RT, D, BF, true, false>), dim3((unsigned)blocks), dim3(256), 0, s, in, out, mask, g, t);
+        hipLaunchKernelGGL((k_line_lowp_dma<RT, D, BF, true, false>), dim3((unsigned)blocks), dim3(256), 0, s, in, out, mask, bits, g, t);
     return hipGetLastError();
 }
 
 template <int RT>
 static hipError_t launch_lowp_staged_rt(const uint16_t *in, uint16_t *out, const uint8_t *mask,
-                                        const LineGeom &g, const float *k, int R, bool bf, hipStream_t s) {
-    const int form = lowp_dma_form(out, mask, g);
+                                        const MaskBits &bits, const LineGeom &g, const float *k, int R,
+                                        bool bf, hipStream_t s) {
+    const int form = lowp_dma_form(out, mask, bits, g);
     if (form != 0 && !force_vgpr_staging()) {
-        if (bf) return launch_lowp_dma_rt<RT, true>(form, in, out, mask, g, k, R, s);
-        return launch_lowp_dma_rt<RT, false>(form, in, out, mask, g, k, R, s);
+        if (bf) return launch_lowp_dma_rt<RT, true>(form, in, out, mask, bits, g, k, R, s);
+        return launch_lowp_dma_rt<RT, false>(form, in, out, mask, bits, g, k, R, s);
     }
     return launch_lowp_lds_rt<RT>(in, out, mask, g, k, R, bf, s);
 }
@@ -552,10 +555,10 @@ bool lowp_lds_supported(const uint16_t *in, const LineGeom &g, int R) {
 }
 
 hipError_t launch_line_pass_lowp_lds(const uint16_t *in, uint16_t *out, const uint8_t *mask,
-                                     const LineGeom &g, const float *taps, int R, bool bf,
-                                     hipStream_t s) {
+                                     const MaskBits &bits, const LineGeom &g, const float *taps,
+                                     int R, bool bf, hipStream_t s) {
 #define MXB_LOWP_LDS_CASE(rt) \
-    if (R <= rt) return launch_lowp_staged_rt<rt>(in, out, mask, g, taps, R, bf, s);
+    if (R <= rt) return launch_lowp_staged_rt<rt>(in, out, mask, bits, g, taps, R, bf, s);
     // the same radius buckets as the register-ring kernel: the bucket fixes
     // how taps group into 32-position chunks, so both forms agree bit for bit
     MXB_LOWP_LDS_CASE(4)

@@ -152,7 +152,7 @@ TapsL<2 * RT + 1> split_taps(const float *k, int R, bool bf) {
 // 16-byte aligned and the radius is compiled.
 bool lowp_lds_supported(const uint16_t *in, const LineGeom &g, int R);
 hipError_t launch_line_pass_lowp_lds(const uint16_t *in, uint16_t *out, const uint8_t *mask,
-                                     const LineGeom &g, const float *taps, int R, bool bf,
-                                     hipStream_t s);
+                                     const MaskBits &bits, const LineGeom &g, const float *taps,
+                                     int R, bool bf, hipStream_t s);
 
 }  // namespace mxb

@@ -27,8 +27,10 @@ def test_library_exports_every_declared_symbol():
 
 def test_state_bytes():
     L = capi.lib()
-    assert L.mxb_state_bytes(capi.PRECISION["fp32"], 4, 5, 6) == 480
-    assert L.mxb_state_bytes(capi.PRECISION["bf16"], 4, 5, 6) == 480  # FP32 last repetition
+    # 480 bytes of FP32 state rounded to 512, then 4 * 5 rows of one bit word
+    assert L.mxb_state_bytes(capi.PRECISION["fp32"], 4, 5, 6) == 512 + 80
+    assert L.mxb_state_bytes(capi.PRECISION["bf16"], 4, 5, 6) == 512 + 80  # FP32 last repetition
+    assert L.mxb_state_bytes(capi.PRECISION["fp32"], 2, 3, 33) == (2 * 3 * 33 * 4 + 255) // 256 * 256 + 2 * 3 * 2 * 4
     assert L.mxb_state_bytes(1024, 4, 5, 6) == -1   # unknown precision
     assert L.mxb_state_bytes(capi.PRECISION["fp32"], 4, -5, 6) == -1
 
