@@ -35,7 +35,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "batch", "fields"])
+    p.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "batch", "fields", "edt"])
     p.add_argument("--precision", default="fp32",
                    choices=["fp32", "bf16", "fp16", "fp32_fma", "fp32_exact"])
     p.add_argument("--reps", type=int, default=None, help="override the workload's repetitions")
@@ -299,6 +299,91 @@ def fields_cpu_baseline(vox, gauss, edt, fld, cyl, vb, fbins, vrange, frange):
                       f"{t['axis_histograms']:.2f} s (compiled CPU histograms module)"}
 
 
+def synth_implant(shape):
+    """uint8 implant_solid mask: a threaded rod along z (radius 0.12 of the
+    slice, modulated by a helical thread) from a quarter of the depth on, and a
+    sparse scatter of bright specks, generated slice by slice."""
+    nz, ny, nx = shape
+    rng = np.random.default_rng(77)
+    yy, xx = np.meshgrid(np.arange(ny) - ny / 2, np.arange(nx) - nx / 2, indexing="ij")
+    r = np.hypot(yy, xx) / (min(ny, nx) / 2)
+    phi = np.arctan2(yy, xx)
+    m = np.zeros(shape, np.uint8)
+    for z in range(nz // 4, nz):
+        thread = 0.015 * np.sin(phi + 2 * np.pi * z / 48.0)
+        m[z] = r < 0.12 + thread
+    flat = m.reshape(-1)
+    flat[rng.integers(0, flat.size, flat.size // 200000)] = 1
+    return m
+
+
+def bench_edt(args):
+    """The EDT field of 0950 (EDT of the negated implant mask, cylinder mask,
+    uint16 conversion) on a 1024^3 implant mask resident in HBM; one step =
+    one mxb_device_edt_field call."""
+    from maxibone_amd import capi, hip
+    from maxibone_amd.helpers import generate_cylinder_mask
+
+    L = capi.lib()
+    shape = (1024, 1024, 1024)
+    nz, ny, nx = shape
+    n = nz * ny * nx
+    implant = synth_implant(shape)
+    cyl = generate_cylinder_mask(ny, nx).astype(np.uint8)
+    d_m, d_c = hip.DeviceBuffer.from_array(implant), hip.DeviceBuffer.from_array(cyl)
+    d_out = hip.DeviceBuffer(2 * n)
+    ws = hip.DeviceBuffer(L.mxb_edt_workspace_bytes(*shape))
+    st = hip.Stream()
+
+    def run():
+        capi.check(L.mxb_device_edt_field(d_m.ptr, *shape, d_c.ptr, d_out.ptr, ws.ptr, st.handle))
+
+    for _ in range(args.warmup):
+        run()
+    st.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(st)
+    for _ in range(args.steps):
+        run()
+    e1.record(st)
+    ms = e0.elapsed_ms(e1) / args.steps
+    # algorithmic HBM bytes per voxel: the mask (1), three passes of uint32
+    # distances (pass x writes 4, y reads and writes 8, z reads 4 and writes the
+    # float field 4), the uint16 conversion (reads 4, writes 2)
+    alg_bytes = 27.0
+    gbps = alg_bytes * n / ms / 1e6
+    rec = {
+        "metric": "Gvoxel/s (EDT field: distance transform + cylinder mask + uint16)",
+        "value": round(n / ms / 1e6, 3),
+        "unit": "Gvoxel/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": "1024^3 implant mask (threaded rod along z from z=256, radius 0.12 of "
+                               "the slice, scattered specks), 0950 EDT field",
+                   "shape": list(shape), "implant_voxels": int(implant.sum(dtype=np.int64))},
+        "roofline": {"bound": "hbm", "achieved": round(gbps, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(gbps / PEAK_HBM_GBS, 4), "traffic": None,
+                     "kernel": "k_edt_pass (3 passes) + k_edt_to_u16"},
+    }
+    for b in (d_m, d_c, d_out, ws):
+        b.free()
+    rec["cpu_baseline"] = edt_cpu_baseline() if args.cpu_seconds > 0 else None
+    print(json.dumps(rec), flush=True)
+
+
+def edt_cpu_baseline():
+    """The oracle's exact EDT (one host thread: the 1D passes of the `edt`
+    package's algorithm in C) on a 256^3 mask of the same construction."""
+    import oracle.edt as OE
+    m = synth_implant((256, 256, 256))
+    t0 = time.perf_counter()
+    OE.edt_sq(m == 0)
+    t = time.perf_counter() - t0
+    return {"value": round(m.size / t / 1e9, 4), "unit": "Gvoxel/s", "cores": 1, "kind": "port",
+            "sample": f"256^3 implant mask of the same construction, squared EDT in {t:.2f} s"}
+
+
 def load_traffic(workload_name, precision, kernel):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -324,6 +409,9 @@ def main():
         return
     if args.workload == "fields":
         bench_fields(args)
+        return
+    if args.workload == "edt":
+        bench_edt(args)
         return
     from maxibone_amd import capi, hip
     from maxibone_amd.synth import WORKLOADS

@@ -13,7 +13,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libmxb_diffusion.so")
 HEADERS = [os.path.join(os.path.dirname(PKG), "include", h)
-           for h in ("maxibone_diffusion.h", "maxibone_fields.h")]
+           for h in ("maxibone_diffusion.h", "maxibone_fields.h", "maxibone_edt.h")]
 
 OK, EINVAL, EDEVICE, ENOMEM, EIO = 0, 1, 2, 3, 4
 PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "fp32_fma": 3, "fp32_exact": 4}
@@ -78,6 +78,10 @@ def lib():
             "mxb_device_axis_histograms": [_vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _i64,
                                            _i64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double,
                                            ctypes.c_double, _vp, _vp],
+            "mxb_edt": [_vp, _i64, _i64, _i64, _vp, _vp, ctypes.c_int],
+            "mxb_edt_field": [_vp, _i64, _i64, _i64, _vp, _vp, ctypes.c_int],
+            "mxb_device_edt": [_vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp],
+            "mxb_device_edt_field": [_vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -89,6 +93,8 @@ def lib():
         L.mxb_combine_workspace_bytes.restype = _i64
         L.mxb_histogram_workspace_bytes.argtypes = [_i64, _i64, _i64]
         L.mxb_histogram_workspace_bytes.restype = _i64
+        L.mxb_edt_workspace_bytes.argtypes = [_i64, _i64, _i64]
+        L.mxb_edt_workspace_bytes.restype = _i64
         L.mxb_last_error.restype = ctypes.c_char_p
         L.mxb_version.restype = ctypes.c_char_p
         _LIB = L

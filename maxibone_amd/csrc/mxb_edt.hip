@@ -1,0 +1,577 @@
+// Euclidean distance transform of a binary volume (gfx950): the core of the
+// implant EDT field (src/processing_steps/0950_generate_edt.py:44-66, which
+// calls edt.edt of the external `edt` package).  C ABI in
+// include/maxibone_edt.h.
+//
+// Squared distances separably, three passes over (z, y, x) volumes:
+//   pass x: g1[i] = min over sites j of the row of (i - j)^2
+//   pass y: g2[i] = min_j g1[j] + (i - j)^2 along y
+//   pass z: g3[i] = min_j g2[j] + (i - j)^2 along z
+// Each step is the lower envelope of the parabolas g[j] + (i - j)^2.  Since
+// g[j] + (i - j)^2 = i^2 + (c_j - 2 i j) with c_j = g[j] + j^2, the minimum
+// over j is attained at a vertex of the lower convex hull of the points
+// (j, c_j), and the minimising vertex moves right as i grows.  (The x pass is
+// the same step with g = 0 at sites and no point elsewhere.)  All arithmetic
+// is exact in 64-bit integers; the distances are integers below 3 * 4096^2.
+//
+// One wave per line (64 lanes, band of L = ceil(n / 64) positions per lane):
+//   A. each lane builds the lower hull of its band (monotone chain; the stack
+//      in LDS);
+//   B. six rounds merge adjacent groups of bands: the lower bridge of two
+//      x-separated hulls is found by walking the left end left and the right
+//      end right until neither moves; the merged hull is a prefix of the left
+//      hull and a suffix of the right one (each band keeps a [lo, hi) range);
+//   C. the surviving vertices are compacted (wave prefix sum);
+//   D. each lane answers its band: a binary search for its first position,
+//      then a forward walk; results stay in registers until the whole line is
+//      answered, then overwrite the line in LDS.
+// A block of 4 waves loads LINES lines into an LDS tile with coalesced loads
+// (for the y and z passes, whose positions are strided, LINES consecutive x
+// per position), works on them in place and stores the tile back the same
+// way.  Tiles are dealt to the 8 XCDs in contiguous ranges, so neighbouring
+// tiles, which share cache lines, meet in one L2.  Positions in a tile line
+// are padded by one word per 16 (p + p / 16) so that lanes stepping through
+// their bands hit different banks.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <string>
+
+#include "maxibone_diffusion.h"
+#include "maxibone_edt.h"
+#include "mxb_internal.h"
+
+namespace mxb {
+
+namespace {
+
+constexpr uint32_t kNone = MXB_EDT_NONE;
+constexpr int kWaves = 4;
+constexpr int kMetaInts = 3 * 64;         // per wave: band lo, hi and spare
+constexpr int kTileBudget = 36 * 1024;    // LDS bytes for the tile of a strided pass
+
+struct EdtPass {
+    const void *in;        // uint8 (first pass) or uint32
+    uint32_t *out;         // uint32, or float bits (last pass)
+    int64_t n;             // positions per line
+    int64_t pos_stride;    // elements between positions
+    int64_t n_inner, inner_stride;
+    int64_t n_outer, outer_stride;
+    int lines;             // lines per block (power of two)
+    int pitch;             // LDS words per tile line
+    const uint8_t *cyl;    // last pass: cylinder mask indexed by the line, or null
+    uint32_t *minmax;      // last pass: ordered-encoded min and max
+};
+
+__device__ __forceinline__ int tix(int p) { return p + (p >> 4); }
+
+// LDS ordering between the lanes of one wave (each wave's LDS operations
+// execute in order; this keeps the compiler from moving them across).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t f2ord_d(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f_d(uint32_t o) {
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+// cross((x0,c0), (x1,c1), (x2,c2)) <= 0: the middle point is on or above the
+// segment between the outer two (x0 < x1 < x2).
+__device__ __forceinline__ bool not_below(int64_t x0, int64_t c0, int64_t x1, int64_t c1, int64_t x2,
+                                          int64_t c2) {
+    return (x1 - x0) * (c2 - c0) - (c1 - c0) * (x2 - x0) <= 0;
+}
+
+// MODE 0: first pass, sites where data == 0; 1: first pass, sites where
+// data != 0; 2: uint32 -> uint32; 3: uint32 -> float distance x cylinder,
+// with the block's min and max folded into P.minmax.
+template <int L, int MODE>
+__global__ __launch_bounds__(256) void k_edt_pass(EdtPass P) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t smem[];
+    const int n = (int)P.n;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    uint32_t *const tile = smem;
+    uint16_t *const stk = reinterpret_cast<uint16_t *>(smem + P.lines * P.pitch) + wv * 2 * n;
+    uint16_t *const env = stk + n;
+    int *const blo = reinterpret_cast<int *>(smem + P.lines * P.pitch + 4 * n) + wv * kMetaInts;
+    int *const bhi = blo + 64;
+
+    // tile of this block (XCD-aware order: block b runs on XCD b % 8)
+    const int64_t chunks = (P.n_inner + P.lines - 1) / P.lines;
+    const int64_t B = chunks * P.n_outer;
+    const int64_t b = blockIdx.x, q = B / 8, rem = B % 8, xcd = b % 8;
+    const int64_t t_id = xcd * q + (xcd < rem ? xcd : rem) + b / 8;
+    const int64_t outer = t_id / chunks, inner0 = (t_id % chunks) * P.lines;
+    const int nl = (int)(P.n_inner - inner0 < P.lines ? P.n_inner - inner0 : P.lines);
+    const int64_t base = outer * P.outer_stride + inner0 * P.inner_stride;
+    const bool lines_contig = P.inner_stride == 1 && P.pos_stride != 1;
+    int lshift = 0;
+    while ((1 << lshift) < P.lines) ++lshift;
+
+    // ---- load
+    if (lines_contig) {
+        const int total = P.lines * n;
+        for (int t = tid; t < total; t += 256) {
+            const int l = t & (P.lines - 1), p = t >> lshift;
+            if (l >= nl) continue;
+            const int64_t o = base + l + (int64_t)p * P.pos_stride;
+            uint32_t v;
+            if (MODE == 0) v = static_cast<const uint8_t *>(P.in)[o] == 0 ? 0u : kNone;
+            else if (MODE == 1) v = static_cast<const uint8_t *>(P.in)[o] != 0 ? 0u : kNone;
+            else v = static_cast<const uint32_t *>(P.in)[o];
+            tile[l * P.pitch + tix(p)] = v;
+        }
+    } else {
+        for (int l = 0; l < nl; ++l)
+            for (int p = tid; p < n; p += 256) {
+                const int64_t o = base + (int64_t)l * P.inner_stride + (int64_t)p * P.pos_stride;
+                uint32_t v;
+                if (MODE == 0) v = static_cast<const uint8_t *>(P.in)[o] == 0 ? 0u : kNone;
+                else if (MODE == 1) v = static_cast<const uint8_t *>(P.in)[o] != 0 ? 0u : kNone;
+                else v = static_cast<const uint32_t *>(P.in)[o];
+                tile[l * P.pitch + tix(p)] = v;
+            }
+    }
+    __syncthreads();
+
+    uint32_t omin = 0xffffffffu, omax = 0u;
+    const int b0 = lane * L;
+    for (int l = wv; l < nl; l += kWaves) {
+        uint32_t *const g = tile + l * P.pitch;
+        auto cval = [&](int64_t x) -> int64_t { return (int64_t)g[tix((int)x)] + x * x; };
+
+        // A. lower hull of the band's points
+        int h = 0;
+        int64_t xa = 0, ca = 0, xb = 0, cb = 0;  // second and top of the stack
+        for (int t = 0; t < L; ++t) {
+            const int j = b0 + t;
+            if (j >= n) break;
+            const uint32_t gj = g[tix(j)];
+            if (gj == kNone) continue;
+            const int64_t cj = (int64_t)gj + (int64_t)j * j;
+            while (h >= 2 && not_below(xa, ca, xb, cb, j, cj)) {
+                --h;
+                xb = xa;
+                cb = ca;
+                if (h >= 2) {
+                    xa = stk[b0 + h - 2];
+                    ca = cval(xa);
+                }
+            }
+            stk[b0 + h] = (uint16_t)j;
+            if (h >= 1) {
+                xa = xb;
+                ca = cb;
+            }
+            xb = j;
+            cb = cj;
+            ++h;
+        }
+        blo[lane] = 0;
+        bhi[lane] = h;
+        wave_sync();
+
+        // B. merge groups of bands [l0, mid) and [mid, l0 + gs)
+        for (int gs = 2; gs <= 64; gs <<= 1) {
+            if ((lane & (gs - 1)) == 0) {
+                const int l0 = lane, mid = lane + gs / 2, r = lane + gs;
+                int ba = mid - 1, bb = mid;
+                while (ba >= l0 && blo[ba] >= bhi[ba]) --ba;
+                while (bb < r && blo[bb] >= bhi[bb]) ++bb;
+                if (ba >= l0 && bb < r) {
+                    int ia = bhi[ba] - 1, ib = blo[bb];
+                    int64_t ax = stk[ba * L + ia], bx = stk[bb * L + ib];
+                    int64_t ac = cval(ax), bc = cval(bx);
+                    for (;;) {
+                        bool moved = false;
+                        for (;;) {  // left end moves left while it is not below (prev, b)
+                            int pa = ba, pi = ia - 1;
+                            if (pi < blo[pa]) {
+                                do { --pa; } while (pa >= l0 && blo[pa] >= bhi[pa]);
+                                if (pa < l0) break;
+                                pi = bhi[pa] - 1;
+                            }
+                            const int64_t px = stk[pa * L + pi], pc = cval(px);
+                            if (!not_below(px, pc, ax, ac, bx, bc)) break;
+                            ba = pa; ia = pi; ax = px; ac = pc;
+                            moved = true;
+                        }
+                        for (;;) {  // right end moves right while it is not below (a, next)
+                            int na = bb, ni = ib + 1;
+                            if (ni >= bhi[na]) {
+                                do { ++na; } while (na < r && blo[na] >= bhi[na]);
+                                if (na >= r) break;
+                                ni = blo[na];
+                            }
+                            const int64_t qx = stk[na * L + ni], qc = cval(qx);
+                            if (!not_below(ax, ac, bx, bc, qx, qc)) break;
+                            bb = na; ib = ni; bx = qx; bc = qc;
+                            moved = true;
+                        }
+                        if (!moved) break;
+                    }
+                    bhi[ba] = ia + 1;
+                    for (int k = ba + 1; k < mid; ++k) bhi[k] = blo[k];
+                    blo[bb] = ib;
+                    for (int k = mid; k < bb; ++k) blo[k] = bhi[k];
+                }
+            }
+            wave_sync();
+        }
+
+        // C. compact the hull
+        const int lo = blo[lane], cnt = bhi[lane] - lo;
+        int incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += v;
+        }
+        const int H = __shfl(incl, 63, 64);
+        const int off = incl - cnt;
+        for (int k = 0; k < cnt; ++k) env[off + k] = stk[b0 + lo + k];
+        wave_sync();
+
+        // D. answers for the band's positions
+        uint32_t res[L];
+        if (H == 0) {
+#pragma unroll
+            for (int t = 0; t < L; ++t) res[t] = kNone;
+        } else {
+            int lo_m = 0, hi_m = H - 1;
+            const int64_t i0 = b0;
+            while (lo_m < hi_m) {  // first m with slope(m, m + 1) >= 2 i0
+                const int mid = (lo_m + hi_m) >> 1;
+                const int64_t x0 = env[mid], x1 = env[mid + 1];
+                if (cval(x1) - cval(x0) >= 2 * i0 * (x1 - x0))
+                    hi_m = mid;
+                else
+                    lo_m = mid + 1;
+            }
+            int m = lo_m;
+            int64_t xm = env[m];
+            uint32_t gm = g[tix((int)xm)];
+            int64_t cm = (int64_t)gm + xm * xm;
+#pragma unroll
+            for (int t = 0; t < L; ++t) {
+                const int64_t i = b0 + t;
+                if (i < n) {
+                    while (m < H - 1) {
+                        const int64_t x1 = env[m + 1];
+                        const uint32_t g1 = g[tix((int)x1)];
+                        const int64_t c1 = (int64_t)g1 + x1 * x1;
+                        if (c1 - cm >= 2 * i * (x1 - xm)) break;
+                        ++m; xm = x1; gm = g1; cm = c1;
+                    }
+                    res[t] = gm + (uint32_t)((i - xm) * (i - xm));
+                } else {
+                    res[t] = kNone;
+                }
+            }
+        }
+        wave_sync();
+        float cyl = 1.0f;
+        if (MODE == 3 && P.cyl != nullptr) cyl = P.cyl[inner0 + l] != 0 ? 1.0f : 0.0f;
+#pragma unroll
+        for (int t = 0; t < L; ++t) {
+            const int i = b0 + t;
+            if (i < n) {
+                if (MODE == 3) {
+                    const float d = res[t] == kNone ? INFINITY : sqrtf((float)res[t]);
+                    const float f = d * cyl;
+                    const uint32_t o = f2ord_d(f);
+                    omin = min(omin, o);
+                    omax = max(omax, o);
+                    g[tix(i)] = __float_as_uint(f);
+                } else {
+                    g[tix(i)] = res[t];
+                }
+            }
+        }
+        wave_sync();
+    }
+    __syncthreads();
+
+    // ---- store
+    if (lines_contig) {
+        const int total = P.lines * n;
+        for (int t = tid; t < total; t += 256) {
+            const int l = t & (P.lines - 1), p = t >> lshift;
+            if (l >= nl) continue;
+            P.out[base + l + (int64_t)p * P.pos_stride] = tile[l * P.pitch + tix(p)];
+        }
+    } else {
+        for (int l = 0; l < nl; ++l)
+            for (int p = tid; p < n; p += 256)
+                P.out[base + (int64_t)l * P.inner_stride + (int64_t)p * P.pos_stride] = tile[l * P.pitch + tix(p)];
+    }
+    if (MODE == 3) {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            omin = min(omin, (uint32_t)__shfl_xor((int)omin, d, 64));
+            omax = max(omax, (uint32_t)__shfl_xor((int)omax, d, 64));
+        }
+        if (lane == 0) {
+            atomicMin(P.minmax, omin);
+            atomicMax(P.minmax + 1, omax);
+        }
+    }
+}
+
+__global__ void k_edt_minmax_init(uint32_t *mm) {
+    if (threadIdx.x == 0) {
+        mm[0] = 0xffffffffu;
+        mm[1] = 0u;
+    }
+}
+
+// general.hh:96-118 on the float field: (uint16)((f - vmin) / (vmax - vmin) *
+// 65535) with vmax starting at FLT_MIN (general.hh:78), then x cylinder.
+__global__ __launch_bounds__(256) void k_edt_to_u16(const float *__restrict__ f, int64_t n, int64_t layer,
+                                                    const uint8_t *__restrict__ cyl,
+                                                    const uint32_t *__restrict__ mm,
+                                                    uint16_t *__restrict__ out) {
+    const float vmin = ord2f_d(mm[0]);
+    const float vmax = fmaxf(ord2f_d(mm[1]), FLT_MIN);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint16_t q = (uint16_t)((f[i] - vmin) / (vmax - vmin) * 65535.0f);
+        if (cyl != nullptr && cyl[i % layer] == 0) q = 0;
+        out[i] = q;
+    }
+}
+
+size_t pass_smem(const EdtPass &P) {
+    return (size_t)4 * ((size_t)P.lines * P.pitch + 4 * (size_t)P.n + (size_t)kWaves * kMetaInts);
+}
+
+template <int L, int MODE>
+hipError_t launch_l(const EdtPass &P, hipStream_t s) {
+    const size_t smem = pass_smem(P);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_edt_pass<L, MODE>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    const int64_t blocks = ((P.n_inner + P.lines - 1) / P.lines) * P.n_outer;
+    hipLaunchKernelGGL((k_edt_pass<L, MODE>), dim3((unsigned)blocks), dim3(256), smem, s, P);
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_mode(const EdtPass &P, hipStream_t s) {
+    const int64_t need = (P.n + 63) / 64;
+    if (need <= 1) return launch_l<1, MODE>(P, s);
+    if (need <= 2) return launch_l<2, MODE>(P, s);
+    if (need <= 4) return launch_l<4, MODE>(P, s);
+    if (need <= 8) return launch_l<8, MODE>(P, s);
+    if (need <= 16) return launch_l<16, MODE>(P, s);
+    if (need <= 32) return launch_l<32, MODE>(P, s);
+    return launch_l<64, MODE>(P, s);
+}
+
+// Geometry of the pass along axis a (0 z, 1 y, 2 x) of an (nz, ny, nx) volume.
+EdtPass pass_geom(int a, int64_t nz, int64_t ny, int64_t nx) {
+    EdtPass P{};
+    if (a == 2) {
+        P.n = nx; P.pos_stride = 1;
+        P.n_inner = nz * ny; P.inner_stride = nx; P.n_outer = 1; P.outer_stride = 0;
+    } else if (a == 1) {
+        P.n = ny; P.pos_stride = nx;
+        P.n_inner = nx; P.inner_stride = 1; P.n_outer = nz; P.outer_stride = ny * nx;
+    } else {
+        P.n = nz; P.pos_stride = ny * nx;
+        P.n_inner = ny * nx; P.inner_stride = 1; P.n_outer = 1; P.outer_stride = 0;
+    }
+    P.pitch = (int)(P.n + P.n / 16 + 1);
+    if (P.inner_stride == 1 && P.pos_stride != 1) {
+        int lines = 16;
+        while (lines > 1 && (int64_t)lines * P.pitch * 4 > kTileBudget) lines >>= 1;
+        P.lines = lines;
+    } else {
+        P.lines = kWaves;
+    }
+    return P;
+}
+
+hipError_t run_pass(int mode, EdtPass P, hipStream_t s) {
+    if (P.n_inner * P.n_outer == 0 || P.n == 0) return hipSuccess;
+    switch (mode) {
+        case 0: return launch_mode<0>(P, s);
+        case 1: return launch_mode<1>(P, s);
+        case 2: return launch_mode<2>(P, s);
+        default: return launch_mode<3>(P, s);
+    }
+}
+
+int64_t vol_words_bytes(int64_t n) { return (n * 4 + 255) / 256 * 256; }
+
+#define MXB_EHIP(expr)                                                          \
+    do {                                                                        \
+        hipError_t _e = (expr);                                                 \
+        if (_e != hipSuccess) {                                                 \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(_e));       \
+            return _e == hipErrorOutOfMemory ? ENOMEM_ : EDEVICE;               \
+        }                                                                       \
+    } while (0)
+
+int check_extents(const char *who, int64_t nz, int64_t ny, int64_t nx) {
+    if (nz < 0 || ny < 0 || nx < 0) {
+        set_error(std::string(who) + ": negative extent");
+        return EINVAL_;
+    }
+    if (nz > MXB_EDT_MAX_AXIS || ny > MXB_EDT_MAX_AXIS || nx > MXB_EDT_MAX_AXIS) {
+        set_error(std::string(who) + ": at most " + std::to_string(MXB_EDT_MAX_AXIS) + " voxels per axis");
+        return EINVAL_;
+    }
+    return OK;
+}
+
+// The three passes; the first reads `data` (sites: data == 0, or != 0 when
+// `invert`), the last writes `last_out` in `last_mode` (2: squared, 3: float
+// x cylinder + min/max into mm).
+int device_edt_core(const uint8_t *data, int64_t nz, int64_t ny, int64_t nx, bool invert, uint32_t *a,
+                    uint32_t *b, uint32_t *last_out, int last_mode, const uint8_t *cyl, uint32_t *mm,
+                    hipStream_t s) {
+    EdtPass px = pass_geom(2, nz, ny, nx), py = pass_geom(1, nz, ny, nx), pz = pass_geom(0, nz, ny, nx);
+    px.in = data; px.out = a;
+    py.in = a; py.out = b;
+    pz.in = b; pz.out = last_out; pz.cyl = cyl; pz.minmax = mm;
+    if (last_mode == 3) {
+        hipLaunchKernelGGL(k_edt_minmax_init, dim3(1), dim3(64), 0, s, mm);
+        MXB_EHIP(hipGetLastError());
+    }
+    MXB_EHIP(run_pass(invert ? 1 : 0, px, s));
+    MXB_EHIP(run_pass(2, py, s));
+    MXB_EHIP(run_pass(last_mode, pz, s));
+    return OK;
+}
+
+struct EBuf {
+    void *p = nullptr;
+    ~EBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t n) {
+        hipError_t e = hipMalloc(&p, n > 0 ? n : 1);
+        if (e != hipSuccess) {
+            set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+            return e == hipErrorOutOfMemory ? ENOMEM_ : EDEVICE;
+        }
+        return OK;
+    }
+};
+
+}  // namespace
+
+}  // namespace mxb
+
+using namespace mxb;
+
+extern "C" {
+
+int64_t mxb_edt_workspace_bytes(int64_t nz, int64_t ny, int64_t nx) {
+    if (nz < 0 || ny < 0 || nx < 0) return -1;
+    return 2 * vol_words_bytes(nz * ny * nx) + 256;
+}
+
+int mxb_device_edt(const uint8_t *dev_data, int64_t nz, int64_t ny, int64_t nx, uint32_t *dev_sq_out,
+                   float *dev_out, void *dev_workspace, void *stream) {
+    int rc = check_extents("edt", nz, ny, nx);
+    if (rc != OK) return rc;
+    const int64_t n = nz * ny * nx;
+    if (n == 0) return OK;
+    if (dev_data == nullptr || dev_workspace == nullptr || (dev_sq_out == nullptr && dev_out == nullptr)) {
+        set_error("edt: null buffer");
+        return EINVAL_;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t *a = (uint32_t *)dev_workspace;
+    uint32_t *b = (uint32_t *)((char *)dev_workspace + vol_words_bytes(n));
+    uint32_t *mm = (uint32_t *)((char *)dev_workspace + 2 * vol_words_bytes(n));
+    if (dev_sq_out != nullptr && (rc = device_edt_core(dev_data, nz, ny, nx, false, a, b, dev_sq_out, 2,
+                                                       nullptr, mm, s)) != OK)
+        return rc;
+    if (dev_out != nullptr && (rc = device_edt_core(dev_data, nz, ny, nx, false, a, b, (uint32_t *)dev_out, 3,
+                                                    nullptr, mm, s)) != OK)
+        return rc;
+    return OK;
+}
+
+int mxb_device_edt_field(const uint8_t *dev_implant_solid, int64_t nz, int64_t ny, int64_t nx,
+                         const uint8_t *dev_cylinder, uint16_t *dev_out, void *dev_workspace, void *stream) {
+    int rc = check_extents("edt_field", nz, ny, nx);
+    if (rc != OK) return rc;
+    const int64_t n = nz * ny * nx;
+    if (n == 0) return OK;
+    if (dev_implant_solid == nullptr || dev_out == nullptr || dev_workspace == nullptr) {
+        set_error("edt_field: null buffer");
+        return EINVAL_;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t *a = (uint32_t *)dev_workspace;
+    uint32_t *b = (uint32_t *)((char *)dev_workspace + vol_words_bytes(n));
+    uint32_t *mm = (uint32_t *)((char *)dev_workspace + 2 * vol_words_bytes(n));
+    // 0950:41 negates the implant mask: the sites are the implant voxels
+    if ((rc = device_edt_core(dev_implant_solid, nz, ny, nx, true, a, b, a, 3, dev_cylinder, mm, s)) != OK)
+        return rc;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_edt_to_u16, dim3((unsigned)blocks), dim3(256), 0, s, (const float *)a, n, ny * nx,
+                       dev_cylinder, mm, dev_out);
+    MXB_EHIP(hipGetLastError());
+    return OK;
+}
+
+int mxb_edt(const uint8_t *data, int64_t nz, int64_t ny, int64_t nx, uint32_t *sq_out, float *out,
+            int verbose) {
+    (void)verbose;
+    int rc = check_extents("edt", nz, ny, nx);
+    if (rc != OK) return rc;
+    const int64_t n = nz * ny * nx;
+    if (n == 0) return OK;
+    if (data == nullptr || (sq_out == nullptr && out == nullptr)) {
+        set_error("edt: null buffer");
+        return EINVAL_;
+    }
+    EBuf d, ws, o2, of;
+    if ((rc = d.alloc(n)) || (rc = ws.alloc(mxb_edt_workspace_bytes(nz, ny, nx)))) return rc;
+    if (sq_out != nullptr && (rc = o2.alloc(n * 4))) return rc;
+    if (out != nullptr && (rc = of.alloc(n * 4))) return rc;
+    MXB_EHIP(hipMemcpy(d.p, data, n, hipMemcpyHostToDevice));
+    rc = mxb_device_edt((const uint8_t *)d.p, nz, ny, nx, (uint32_t *)o2.p, (float *)of.p, ws.p, nullptr);
+    if (rc != OK) return rc;
+    if (sq_out != nullptr) MXB_EHIP(hipMemcpy(sq_out, o2.p, n * 4, hipMemcpyDeviceToHost));
+    if (out != nullptr) MXB_EHIP(hipMemcpy(out, of.p, n * 4, hipMemcpyDeviceToHost));
+    return OK;
+}
+
+int mxb_edt_field(const uint8_t *implant_solid, int64_t nz, int64_t ny, int64_t nx, const uint8_t *cylinder,
+                  uint16_t *out, int verbose) {
+    (void)verbose;
+    int rc = check_extents("edt_field", nz, ny, nx);
+    if (rc != OK) return rc;
+    const int64_t n = nz * ny * nx;
+    if (n == 0) return OK;
+    if (implant_solid == nullptr || out == nullptr) {
+        set_error("edt_field: null buffer");
+        return EINVAL_;
+    }
+    EBuf d, ws, o, c;
+    if ((rc = d.alloc(n)) || (rc = ws.alloc(mxb_edt_workspace_bytes(nz, ny, nx))) || (rc = o.alloc(n * 2)))
+        return rc;
+    if (cylinder != nullptr && (rc = c.alloc(ny * nx))) return rc;
+    MXB_EHIP(hipMemcpy(d.p, implant_solid, n, hipMemcpyHostToDevice));
+    if (cylinder != nullptr) MXB_EHIP(hipMemcpy(c.p, cylinder, ny * nx, hipMemcpyHostToDevice));
+    rc = mxb_device_edt_field((const uint8_t *)d.p, nz, ny, nx, (const uint8_t *)c.p, (uint16_t *)o.p, ws.p,
+                              nullptr);
+    if (rc != OK) return rc;
+    MXB_EHIP(hipMemcpy(out, o.p, n * 2, hipMemcpyDeviceToHost));
+    return OK;
+}
+
+}  // extern "C"

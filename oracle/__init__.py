@@ -14,6 +14,8 @@ Contents
   ``core_f32`` / ``quantise_u16``: ctypes wrappers over ``liboracle.so``
   (``diffusion_oracle.c``), which restates src/lib/cpp/cpu/diffusion.cc:31-60,
   191-196, 243-248, 295-302, 314-320, 339-374 and 376-468.
+* ``edt``: the EDT field of src/processing_steps/0950_generate_edt.py
+  (``edt_oracle.c`` for the distance transform; see ``oracle/edt.py``).
 * ``ref_module()``: the reference's own CPU module compiled from its sources
   into ``oracle/_ref`` by ``oracle/build.py`` (None when not built).
 """
@@ -55,6 +57,9 @@ def lib():
         L.oracle_diffusion_chunked_f32.argtypes = [_u8p, _i64, _i64, _i64, _f32p, _i64, _i64,
                                                    _i64, ctypes.c_int, _f32p]
         L.oracle_diffusion_chunked_f32.restype = ctypes.c_int
+        L.oracle_edt_sq.argtypes = [_u8p, _i64, _i64, _i64,
+                                    np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")]
+        L.oracle_edt_sq.restype = ctypes.c_int
         _LIB = L
     return _LIB
 

@@ -1,6 +1,7 @@
 """Build recipe for the oracle (test infrastructure only).
 
-* ``liboracle.so``: the plain-C restatement in ``diffusion_oracle.c``.
+* ``liboracle.so``: the plain-C restatements in ``diffusion_oracle.c`` (the
+  diffusion) and ``edt_oracle.c`` (the EDT field's distance transform).
   Flags follow the reference CPU build (src/Makefile:9-18: -O3, OpenMP, no
   -march) plus ``-ffp-contract=off`` so that no FMA is ever formed.
 * ``_ref/<module><ext-suffix>``: the reference's own CPU modules, compiled
@@ -28,9 +29,9 @@ def _run(cmd):
 
 def build_oracle() -> str:
     out = os.path.join(HERE, "liboracle.so")
-    src = os.path.join(HERE, "diffusion_oracle.c")
-    _run(["gcc", "-O3", "-fopenmp", "-ffp-contract=off", "-std=c11", "-fPIC", "-shared",
-          "-Wall", "-Wextra", src, "-o", out + ".tmp", "-lm"])
+    srcs = [os.path.join(HERE, f) for f in ("diffusion_oracle.c", "edt_oracle.c")]
+    _run(["gcc", "-O3", "-fopenmp", "-ffp-contract=off", "-std=gnu11", "-fPIC", "-shared",
+          "-Wall", "-Wextra", *srcs, "-o", out + ".tmp", "-lm"])
     os.replace(out + ".tmp", out)
     return out
 

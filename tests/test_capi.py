@@ -116,3 +116,19 @@ def test_batched_volume_recipe():
     for v in vols:
         assert all(64 <= s <= 384 for s in v.mask.shape)
         assert 2 <= v.sigma <= 10 and v.radius == round(4 * v.sigma)
+
+
+def test_edt_argument_checks_before_the_device():
+    """The EDT entries reject oversized axes and missing outputs before any
+    device work (include/maxibone_edt.h: at most 4096 voxels per axis)."""
+    from maxibone_amd import edt as E
+    L = capi.lib()
+    assert L.mxb_edt_workspace_bytes(4, 5, 6) == 2 * 512 + 256  # 480 B volumes rounded to 512
+    assert L.mxb_edt_workspace_bytes(-1, 5, 6) == -1
+    big = np.zeros(1, np.uint8)
+    out = np.zeros(1, np.float32)
+    assert L.mxb_edt(big.ctypes.data, E.MAX_AXIS + 1, 1, 1, None, out.ctypes.data, 0) == capi.EINVAL
+    assert L.mxb_edt(big.ctypes.data, 1, 1, 1, None, None, 0) == capi.EINVAL
+    assert L.mxb_edt_field(big.ctypes.data, 1, E.MAX_AXIS + 1, 1, None, None, 0) == capi.EINVAL
+    with pytest.raises(ValueError):
+        E.edt(np.zeros((4, 4), bool))
