@@ -46,9 +46,9 @@ namespace mxb {
 namespace {
 
 constexpr uint32_t kNone = MXB_EDT_NONE;
-constexpr int kWaves = 4;
+constexpr int kMaxLines = 8;              // lines (= waves) per block
 constexpr int kMetaInts = 3 * 64;         // per wave: band lo, hi and spare
-constexpr int kTileBudget = 36 * 1024;    // LDS bytes for the tile of a strided pass
+constexpr int kBlockBudget = 80 * 1024;   // LDS bytes per block (two blocks per CU)
 
 struct EdtPass {
     const void *in;        // uint8 (first pass) or uint32
@@ -82,24 +82,45 @@ __device__ __forceinline__ float ord2f_d(uint32_t o) {
 }
 
 // cross((x0,c0), (x1,c1), (x2,c2)) <= 0: the middle point is on or above the
-// segment between the outer two (x0 < x1 < x2).
-__device__ __forceinline__ bool not_below(int64_t x0, int64_t c0, int64_t x1, int64_t c1, int64_t x2,
-                                          int64_t c2) {
-    return (x1 - x0) * (c2 - c0) - (c1 - c0) * (x2 - x0) <= 0;
+// segment between the outer two (x0 < x1 < x2).  Positions are below 2^12
+// and c = g + x^2 below 3 * 4096^2 < 2^26 + 2^24, so the operands are 32-bit
+// and each product (below 2^39) comes from one 32 x 32 -> 64 multiply.
+__device__ __forceinline__ bool not_below(int x0, int c0, int x1, int c1, int x2, int c2) {
+    return (int64_t)(x1 - x0) * (c2 - c0) - (int64_t)(c1 - c0) * (x2 - x0) <= 0;
+}
+
+// Highest / lowest set bit of a nonzero 64-bit mask.
+__device__ __forceinline__ int top_bit(uint64_t m) { return 63 - __builtin_clzll(m); }
+__device__ __forceinline__ int low_bit(uint64_t m) { return __builtin_ctzll(m); }
+__device__ __forceinline__ uint64_t bits_below(int k) { return k <= 0 ? 0ull : (k >= 64 ? ~0ull : (1ull << k) - 1); }
+
+// Position from which hull vertex m + 1 beats vertex m (they tie at no
+// integer before it): the least i with c1 - c0 < 2 i (x1 - x0), clamped to
+// [0, n].
+__device__ __forceinline__ int switch_pos(int x0, int c0, int x1, int c1, int n) {
+    const int num = c1 - c0;
+    if (num < 0) return 0;
+    const int t = (int)((uint32_t)num / (uint32_t)(2 * (x1 - x0))) + 1;
+    return t > n ? n : t;
 }
 
 // MODE 0: first pass, sites where data == 0; 1: first pass, sites where
 // data != 0; 2: uint32 -> uint32; 3: uint32 -> float distance x cylinder,
 // with the block's min and max folded into P.minmax.
 template <int L, int MODE>
-__global__ __launch_bounds__(256) void k_edt_pass(EdtPass P) {
+__global__ __launch_bounds__(kMaxLines * 64) void k_edt_pass(EdtPass P) {
     extern __shared__ __attribute__((aligned(16))) uint32_t smem[];
+    const int kThreads = P.lines * 64;  // one wave per line
     const int n = (int)P.n;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     uint32_t *const tile = smem;
-    uint16_t *const stk = reinterpret_cast<uint16_t *>(smem + P.lines * P.pitch) + wv * 2 * n;
-    uint16_t *const env = stk + n;
-    int *const blo = reinterpret_cast<int *>(smem + P.lines * P.pitch + 4 * n) + wv * kMetaInts;
+    // per wave: stk (64 L uint16: the band stacks, entry k of band b at
+    // k * 64 + b, later the switch table, position p at tpos(p)), env (64 L
+    // uint16); lanes touch consecutive entries, so no bank conflicts
+    uint16_t *const stk = reinterpret_cast<uint16_t *>(smem + P.lines * P.pitch) + wv * 128 * L;
+    uint16_t *const env = stk + 64 * L;
+    int *const blo = reinterpret_cast<int *>(smem + P.lines * P.pitch + P.lines * 64 * L) + wv * kMetaInts;
+    auto tpos = [](int p) { return (p % L) * 64 + p / L; };
     int *const bhi = blo + 64;
 
     // tile of this block (XCD-aware order: block b runs on XCD b % 8)
@@ -117,7 +138,7 @@ __global__ __launch_bounds__(256) void k_edt_pass(EdtPass P) {
     // ---- load
     if (lines_contig) {
         const int total = P.lines * n;
-        for (int t = tid; t < total; t += 256) {
+        for (int t = tid; t < total; t += kThreads) {
             const int l = t & (P.lines - 1), p = t >> lshift;
             if (l >= nl) continue;
             const int64_t o = base + l + (int64_t)p * P.pos_stride;
@@ -129,7 +150,7 @@ __global__ __launch_bounds__(256) void k_edt_pass(EdtPass P) {
         }
     } else {
         for (int l = 0; l < nl; ++l)
-            for (int p = tid; p < n; p += 256) {
+            for (int p = tid; p < n; p += kThreads) {
                 const int64_t o = base + (int64_t)l * P.inner_stride + (int64_t)p * P.pos_stride;
                 uint32_t v;
                 if (MODE == 0) v = static_cast<const uint8_t *>(P.in)[o] == 0 ? 0u : kNone;
@@ -142,76 +163,89 @@ __global__ __launch_bounds__(256) void k_edt_pass(EdtPass P) {
 
     uint32_t omin = 0xffffffffu, omax = 0u;
     const int b0 = lane * L;
-    for (int l = wv; l < nl; l += kWaves) {
+    for (int l = wv; l < nl; l += P.lines) {
         uint32_t *const g = tile + l * P.pitch;
-        auto cval = [&](int64_t x) -> int64_t { return (int64_t)g[tix((int)x)] + x * x; };
+        float cyl = 1.0f;
+        if (MODE == 3 && P.cyl != nullptr) cyl = P.cyl[inner0 + l] != 0 ? 1.0f : 0.0f;
+        auto cval = [&](int x) -> int { return (int)g[tix(x)] + x * x; };
 
-        // A. lower hull of the band's points
+        // A. lower hull of the band's points (the band's values preloaded)
+        uint32_t gv[L];
+#pragma unroll
+        for (int t = 0; t < L; ++t) gv[t] = b0 + t < n ? g[tix(b0 + t)] : kNone;
         int h = 0;
-        int64_t xa = 0, ca = 0, xb = 0, cb = 0;  // second and top of the stack
+        int xa = 0, ca = 0, xb = 0, cb = 0;  // second and top of the stack
+#pragma unroll
         for (int t = 0; t < L; ++t) {
             const int j = b0 + t;
-            if (j >= n) break;
-            const uint32_t gj = g[tix(j)];
-            if (gj == kNone) continue;
-            const int64_t cj = (int64_t)gj + (int64_t)j * j;
-            while (h >= 2 && not_below(xa, ca, xb, cb, j, cj)) {
-                --h;
-                xb = xa;
-                cb = ca;
-                if (h >= 2) {
-                    xa = stk[b0 + h - 2];
-                    ca = cval(xa);
+            if (gv[t] != kNone) {
+                const int cj = (int)gv[t] + j * j;
+                while (h >= 2 && not_below(xa, ca, xb, cb, j, cj)) {
+                    --h;
+                    xb = xa;
+                    cb = ca;
+                    if (h >= 2) {
+                        xa = stk[(h - 2) * 64 + lane];
+                        ca = cval(xa);
+                    }
                 }
+                stk[h * 64 + lane] = (uint16_t)j;
+                if (h >= 1) {
+                    xa = xb;
+                    ca = cb;
+                }
+                xb = j;
+                cb = cj;
+                ++h;
             }
-            stk[b0 + h] = (uint16_t)j;
-            if (h >= 1) {
-                xa = xb;
-                ca = cb;
-            }
-            xb = j;
-            cb = cj;
-            ++h;
         }
+        int my_lo = 0, my_hi = h;
         blo[lane] = 0;
         bhi[lane] = h;
         wave_sync();
 
-        // B. merge groups of bands [l0, mid) and [mid, l0 + gs)
+        // B. merge groups of bands [l0, mid) and [mid, l0 + gs); the bands'
+        // emptiness at the start of a round comes from one ballot
         for (int gs = 2; gs <= 64; gs <<= 1) {
+            const uint64_t ne = __ballot(my_hi > my_lo);
             if ((lane & (gs - 1)) == 0) {
                 const int l0 = lane, mid = lane + gs / 2, r = lane + gs;
-                int ba = mid - 1, bb = mid;
-                while (ba >= l0 && blo[ba] >= bhi[ba]) --ba;
-                while (bb < r && blo[bb] >= bhi[bb]) ++bb;
-                if (ba >= l0 && bb < r) {
+                const uint64_t left = ne & bits_below(mid) & ~bits_below(l0);
+                const uint64_t right = ne & bits_below(r) & ~bits_below(mid);
+                if (left != 0 && right != 0) {
+                    int ba = top_bit(left), bb = low_bit(right);
                     int ia = bhi[ba] - 1, ib = blo[bb];
-                    int64_t ax = stk[ba * L + ia], bx = stk[bb * L + ib];
-                    int64_t ac = cval(ax), bc = cval(bx);
+                    int ax = stk[ia * 64 + ba], bx = stk[ib * 64 + bb];
+                    int ac = cval(ax), bc = cval(bx);
+                    int lo_a = blo[ba], hi_b = bhi[bb];  // range ends of the current bands
                     for (;;) {
                         bool moved = false;
                         for (;;) {  // left end moves left while it is not below (prev, b)
-                            int pa = ba, pi = ia - 1;
-                            if (pi < blo[pa]) {
-                                do { --pa; } while (pa >= l0 && blo[pa] >= bhi[pa]);
-                                if (pa < l0) break;
+                            int pa = ba, pi = ia - 1, plo = lo_a;
+                            if (pi < lo_a) {
+                                const uint64_t before = left & bits_below(ba);
+                                if (before == 0) break;
+                                pa = top_bit(before);
+                                plo = blo[pa];
                                 pi = bhi[pa] - 1;
                             }
-                            const int64_t px = stk[pa * L + pi], pc = cval(px);
+                            const int px = stk[pi * 64 + pa], pc = cval(px);
                             if (!not_below(px, pc, ax, ac, bx, bc)) break;
-                            ba = pa; ia = pi; ax = px; ac = pc;
+                            ba = pa; ia = pi; ax = px; ac = pc; lo_a = plo;
                             moved = true;
                         }
                         for (;;) {  // right end moves right while it is not below (a, next)
-                            int na = bb, ni = ib + 1;
-                            if (ni >= bhi[na]) {
-                                do { ++na; } while (na < r && blo[na] >= bhi[na]);
-                                if (na >= r) break;
+                            int na = bb, ni = ib + 1, nhi = hi_b;
+                            if (ni >= hi_b) {
+                                const uint64_t after = right & ~bits_below(bb + 1);
+                                if (after == 0) break;
+                                na = low_bit(after);
                                 ni = blo[na];
+                                nhi = bhi[na];
                             }
-                            const int64_t qx = stk[na * L + ni], qc = cval(qx);
+                            const int qx = stk[ni * 64 + na], qc = cval(qx);
                             if (!not_below(ax, ac, bx, bc, qx, qc)) break;
-                            bb = na; ib = ni; bx = qx; bc = qc;
+                            bb = na; ib = ni; bx = qx; bc = qc; hi_b = nhi;
                             moved = true;
                         }
                         if (!moved) break;
@@ -223,10 +257,12 @@ __global__ __launch_bounds__(256) void k_edt_pass(EdtPass P) {
                 }
             }
             wave_sync();
+            my_lo = blo[lane];
+            my_hi = bhi[lane];
         }
 
-        // C. compact the hull
-        const int lo = blo[lane], cnt = bhi[lane] - lo;
+        // C. compact the hull into env
+        const int cnt = my_hi - my_lo;
         int incl = cnt;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -235,49 +271,62 @@ __global__ __launch_bounds__(256) void k_edt_pass(EdtPass P) {
         }
         const int H = __shfl(incl, 63, 64);
         const int off = incl - cnt;
-        for (int k = 0; k < cnt; ++k) env[off + k] = stk[b0 + lo + k];
+        for (int k = 0; k < cnt; ++k) env[off + k] = stk[(my_lo + k) * 64 + lane];
         wave_sync();
 
-        // D. answers for the band's positions
+        // D. minimising vertex of every position: vertex m + 1 takes over from
+        // m at switch_pos(m); the switch table (in the stack's place) holds at
+        // each position the largest m + 1 switching there, and a prefix max
+        // over positions gives each position its vertex
         uint32_t res[L];
         if (H == 0) {
 #pragma unroll
             for (int t = 0; t < L; ++t) res[t] = kNone;
         } else {
-            int lo_m = 0, hi_m = H - 1;
-            const int64_t i0 = b0;
-            while (lo_m < hi_m) {  // first m with slope(m, m + 1) >= 2 i0
-                const int mid = (lo_m + hi_m) >> 1;
-                const int64_t x0 = env[mid], x1 = env[mid + 1];
-                if (cval(x1) - cval(x0) >= 2 * i0 * (x1 - x0))
-                    hi_m = mid;
-                else
-                    lo_m = mid + 1;
+#pragma unroll
+            for (int t = 0; t < L; ++t) stk[t * 64 + lane] = 0;
+            wave_sync();
+            // edge m = 64 k + lane; edge m + 1 is the next lane's (lane 63:
+            // lane 0's of the next chunk); a sentinel n + 1 follows the last edge
+            auto edge_switch = [&](int m) -> int {
+                if (m >= H - 1) return n + 1;
+                const int x0 = env[m], x1 = env[m + 1];
+                return switch_pos(x0, cval(x0), x1, cval(x1), n);
+            };
+            int s_cur = edge_switch(lane);
+            for (int m0 = 0; m0 < H - 1; m0 += 64) {
+                const int s_nxt = edge_switch(m0 + 64 + lane);
+                const int s_dn = __shfl_down(s_cur, 1, 64);
+                const int s_first = __shfl(s_nxt, 0, 64);
+                const int s_next = lane < 63 ? s_dn : s_first;
+                const int m = m0 + lane;
+                if (m < H - 1 && s_next > s_cur && s_cur < n) stk[tpos(s_cur)] = (uint16_t)(m + 1);
+                s_cur = s_nxt;
             }
-            int m = lo_m;
-            int64_t xm = env[m];
-            uint32_t gm = g[tix((int)xm)];
-            int64_t cm = (int64_t)gm + xm * xm;
+            wave_sync();
+            int run = 0;
+            int mv[L];
 #pragma unroll
             for (int t = 0; t < L; ++t) {
-                const int64_t i = b0 + t;
-                if (i < n) {
-                    while (m < H - 1) {
-                        const int64_t x1 = env[m + 1];
-                        const uint32_t g1 = g[tix((int)x1)];
-                        const int64_t c1 = (int64_t)g1 + x1 * x1;
-                        if (c1 - cm >= 2 * i * (x1 - xm)) break;
-                        ++m; xm = x1; gm = g1; cm = c1;
-                    }
-                    res[t] = gm + (uint32_t)((i - xm) * (i - xm));
-                } else {
-                    res[t] = kNone;
-                }
+                run = max(run, (int)stk[t * 64 + lane]);
+                mv[t] = run;
+            }
+            int carry = run;  // exclusive prefix max over lanes
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int v = __shfl_up(carry, d, 64);
+                if (lane >= d) carry = max(carry, v);
+            }
+            carry = __shfl_up(carry, 1, 64);
+            if (lane == 0) carry = 0;
+#pragma unroll
+            for (int t = 0; t < L; ++t) {
+                const int i = b0 + t;
+                const int xm = env[max(carry, mv[t])];
+                res[t] = g[tix(xm)] + (uint32_t)((i - xm) * (i - xm));
             }
         }
         wave_sync();
-        float cyl = 1.0f;
-        if (MODE == 3 && P.cyl != nullptr) cyl = P.cyl[inner0 + l] != 0 ? 1.0f : 0.0f;
 #pragma unroll
         for (int t = 0; t < L; ++t) {
             const int i = b0 + t;
@@ -301,25 +350,39 @@ __global__ __launch_bounds__(256) void k_edt_pass(EdtPass P) {
     // ---- store
     if (lines_contig) {
         const int total = P.lines * n;
-        for (int t = tid; t < total; t += 256) {
+        for (int t = tid; t < total; t += kThreads) {
             const int l = t & (P.lines - 1), p = t >> lshift;
             if (l >= nl) continue;
             P.out[base + l + (int64_t)p * P.pos_stride] = tile[l * P.pitch + tix(p)];
         }
     } else {
         for (int l = 0; l < nl; ++l)
-            for (int p = tid; p < n; p += 256)
+            for (int p = tid; p < n; p += kThreads)
                 P.out[base + (int64_t)l * P.inner_stride + (int64_t)p * P.pos_stride] = tile[l * P.pitch + tix(p)];
     }
     if (MODE == 3) {
+        // block min / max (in the tile's first words, free after the store),
+        // then one atomic per value, only when it improves the global one
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
             omin = min(omin, (uint32_t)__shfl_xor((int)omin, d, 64));
             omax = max(omax, (uint32_t)__shfl_xor((int)omax, d, 64));
         }
+        __syncthreads();
         if (lane == 0) {
-            atomicMin(P.minmax, omin);
-            atomicMax(P.minmax + 1, omax);
+            tile[2 * wv] = omin;
+            tile[2 * wv + 1] = omax;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            for (int w = 1; w < P.lines; ++w) {
+                omin = min(omin, tile[2 * w]);
+                omax = max(omax, tile[2 * w + 1]);
+            }
+            if (omin < __hip_atomic_load(P.minmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMin(P.minmax, omin);
+            if (omax > __hip_atomic_load(P.minmax + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMax(P.minmax + 1, omax);
         }
     }
 }
@@ -347,8 +410,16 @@ __global__ __launch_bounds__(256) void k_edt_to_u16(const float *__restrict__ f,
     }
 }
 
+// Band length per lane: the least power of two with 64 L >= n.
+int64_t band_of(int64_t n) {
+    int64_t L = 1;
+    while (64 * L < n) L <<= 1;
+    return L;
+}
+
 size_t pass_smem(const EdtPass &P) {
-    return (size_t)4 * ((size_t)P.lines * P.pitch + 4 * (size_t)P.n + (size_t)kWaves * kMetaInts);
+    const int64_t L = band_of(P.n);
+    return (size_t)4 * P.lines * ((size_t)P.pitch + (size_t)(64 * L) + kMetaInts);
 }
 
 template <int L, int MODE>
@@ -358,7 +429,7 @@ hipError_t launch_l(const EdtPass &P, hipStream_t s) {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
     if (e != hipSuccess) return e;
     const int64_t blocks = ((P.n_inner + P.lines - 1) / P.lines) * P.n_outer;
-    hipLaunchKernelGGL((k_edt_pass<L, MODE>), dim3((unsigned)blocks), dim3(256), smem, s, P);
+    hipLaunchKernelGGL((k_edt_pass<L, MODE>), dim3((unsigned)blocks), dim3(P.lines * 64), smem, s, P);
     return hipGetLastError();
 }
 
@@ -388,13 +459,10 @@ EdtPass pass_geom(int a, int64_t nz, int64_t ny, int64_t nx) {
         P.n_inner = ny * nx; P.inner_stride = 1; P.n_outer = 1; P.outer_stride = 0;
     }
     P.pitch = (int)(P.n + P.n / 16 + 1);
-    if (P.inner_stride == 1 && P.pos_stride != 1) {
-        int lines = 16;
-        while (lines > 1 && (int64_t)lines * P.pitch * 4 > kTileBudget) lines >>= 1;
-        P.lines = lines;
-    } else {
-        P.lines = kWaves;
-    }
+    int lines = kMaxLines;  // one wave per line; the largest count within the block budget
+    while (lines > 1 && (int64_t)4 * lines * (P.pitch + 64 * band_of(P.n) + kMetaInts) > kBlockBudget)
+        lines >>= 1;
+    P.lines = lines;
     return P;
 }
 
