@@ -7,31 +7,40 @@
 //   pass x: g1[i] = min over sites j of the row of (i - j)^2
 //   pass y: g2[i] = min_j g1[j] + (i - j)^2 along y
 //   pass z: g3[i] = min_j g2[j] + (i - j)^2 along z
-// Each step is the lower envelope of the parabolas g[j] + (i - j)^2.  Since
-// g[j] + (i - j)^2 = i^2 + (c_j - 2 i j) with c_j = g[j] + j^2, the minimum
-// over j is attained at a vertex of the lower convex hull of the points
-// (j, c_j), and the minimising vertex moves right as i grows.  (The x pass is
-// the same step with g = 0 at sites and no point elsewhere.)  All arithmetic
-// is exact in 64-bit integers; the distances are integers below 3 * 4096^2.
+// Pass x is a row scan (k_edt_rows: per 64-position chunk a ballot of the
+// sites, then the nearest site before and after each chunk by prefix max /
+// suffix min over the chunks).
 //
-// One wave per line (64 lanes, band of L = ceil(n / 64) positions per lane):
-//   A. each lane builds the lower hull of its band (monotone chain; the stack
-//      in LDS);
+// Passes y and z are the lower envelope of the parabolas g[j] + (i - j)^2.
+// Since g[j] + (i - j)^2 = i^2 + (c_j - 2 i j) with c_j = g[j] + j^2, the
+// minimum over j is attained at a vertex of the lower convex hull of the
+// points (j, c_j), and the minimising vertex moves right as i grows.  With
+// at most 4096 voxels per axis, positions are below 2^12 and c below 2^27:
+// 32-bit values, with 64-bit cross products.  One wave per line (64 lanes,
+// a band of L positions per lane, 64 L >= n):
+//   A. each lane builds the lower hull of its band (monotone chain; the
+//      band's values preloaded into registers, the stack in LDS);
 //   B. six rounds merge adjacent groups of bands: the lower bridge of two
 //      x-separated hulls is found by walking the left end left and the right
-//      end right until neither moves; the merged hull is a prefix of the left
-//      hull and a suffix of the right one (each band keeps a [lo, hi) range);
+//      end right until neither moves (the bands' emptiness from one ballot
+//      per round); the merged hull is a prefix of the left hull and a suffix
+//      of the right one (each band keeps a [lo, hi) range);
 //   C. the surviving vertices are compacted (wave prefix sum);
-//   D. each lane answers its band: a binary search for its first position,
-//      then a forward walk; results stay in registers until the whole line is
-//      answered, then overwrite the line in LDS.
-// A block of 4 waves loads LINES lines into an LDS tile with coalesced loads
-// (for the y and z passes, whose positions are strided, LINES consecutive x
-// per position), works on them in place and stores the tile back the same
-// way.  Tiles are dealt to the 8 XCDs in contiguous ranges, so neighbouring
-// tiles, which share cache lines, meet in one L2.  Positions in a tile line
-// are padded by one word per 16 (p + p / 16) so that lanes stepping through
-// their bands hit different banks.
+//   D. vertex m + 1 beats vertex m from an integer switch position on, and
+//      the switch positions grow with m: a table holding at each position the
+//      largest m + 1 switching there, then a prefix max over the positions,
+//      gives every position its vertex.
+// The results stay in registers until the line is answered, then overwrite
+// the line in LDS.
+//
+// A block has one wave per line and holds its lines in an LDS tile, loaded
+// and stored with coalesced accesses (for the strided passes, consecutive x
+// per position); the line count is the largest that keeps a block within
+// kBlockBudget bytes of LDS.  Tiles are dealt to the 8 XCDs in contiguous
+// ranges, so neighbouring tiles, which share cache lines, meet in one L2.
+// Positions in a tile line are padded by one word per 16 (p + p / 16), and
+// the band stacks are stored entry-major (entry k of band b at k * 64 + b),
+// so that lanes hit different banks.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -104,10 +113,9 @@ __device__ __forceinline__ int switch_pos(int x0, int c0, int x1, int c1, int n)
     return t > n ? n : t;
 }
 
-// MODE 0: first pass, sites where data == 0; 1: first pass, sites where
-// data != 0; 2: uint32 -> uint32; 3: uint32 -> float distance x cylinder,
-// with the block's min and max folded into P.minmax.
-template <int L, int MODE>
+// Envelope pass along y or z: uint32 -> uint32, or (LAST) uint32 -> float
+// distance x cylinder with the block's min and max folded into P.minmax.
+template <int L, bool LAST>
 __global__ __launch_bounds__(kMaxLines * 64) void k_edt_pass(EdtPass P) {
     extern __shared__ __attribute__((aligned(16))) uint32_t smem[];
     const int kThreads = P.lines * 64;  // one wave per line
@@ -142,21 +150,13 @@ __global__ __launch_bounds__(kMaxLines * 64) void k_edt_pass(EdtPass P) {
             const int l = t & (P.lines - 1), p = t >> lshift;
             if (l >= nl) continue;
             const int64_t o = base + l + (int64_t)p * P.pos_stride;
-            uint32_t v;
-            if (MODE == 0) v = static_cast<const uint8_t *>(P.in)[o] == 0 ? 0u : kNone;
-            else if (MODE == 1) v = static_cast<const uint8_t *>(P.in)[o] != 0 ? 0u : kNone;
-            else v = static_cast<const uint32_t *>(P.in)[o];
-            tile[l * P.pitch + tix(p)] = v;
+            tile[l * P.pitch + tix(p)] = static_cast<const uint32_t *>(P.in)[o];
         }
     } else {
         for (int l = 0; l < nl; ++l)
             for (int p = tid; p < n; p += kThreads) {
                 const int64_t o = base + (int64_t)l * P.inner_stride + (int64_t)p * P.pos_stride;
-                uint32_t v;
-                if (MODE == 0) v = static_cast<const uint8_t *>(P.in)[o] == 0 ? 0u : kNone;
-                else if (MODE == 1) v = static_cast<const uint8_t *>(P.in)[o] != 0 ? 0u : kNone;
-                else v = static_cast<const uint32_t *>(P.in)[o];
-                tile[l * P.pitch + tix(p)] = v;
+                tile[l * P.pitch + tix(p)] = static_cast<const uint32_t *>(P.in)[o];
             }
     }
     __syncthreads();
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(kMaxLines * 64) void k_edt_pass(EdtPass P) {
     for (int l = wv; l < nl; l += P.lines) {
         uint32_t *const g = tile + l * P.pitch;
         float cyl = 1.0f;
-        if (MODE == 3 && P.cyl != nullptr) cyl = P.cyl[inner0 + l] != 0 ? 1.0f : 0.0f;
+        if (LAST && P.cyl != nullptr) cyl = P.cyl[inner0 + l] != 0 ? 1.0f : 0.0f;
         auto cval = [&](int x) -> int { return (int)g[tix(x)] + x * x; };
 
         // A. lower hull of the band's points (the band's values preloaded)
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(kMaxLines * 64) void k_edt_pass(EdtPass P) {
         for (int t = 0; t < L; ++t) {
             const int i = b0 + t;
             if (i < n) {
-                if (MODE == 3) {
+                if (LAST) {
                     const float d = res[t] == kNone ? INFINITY : sqrtf((float)res[t]);
                     const float f = d * cyl;
                     const uint32_t o = f2ord_d(f);
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(kMaxLines * 64) void k_edt_pass(EdtPass P) {
             for (int p = tid; p < n; p += kThreads)
                 P.out[base + (int64_t)l * P.inner_stride + (int64_t)p * P.pos_stride] = tile[l * P.pitch + tix(p)];
     }
-    if (MODE == 3) {
+    if (LAST) {
         // block min / max (in the tile's first words, free after the store),
         // then one atomic per value, only when it improves the global one
 #pragma unroll
@@ -394,19 +394,104 @@ __global__ void k_edt_minmax_init(uint32_t *mm) {
     }
 }
 
+// Pass x: g1 = squared distance to the nearest site of the row (sites:
+// data == 0, or data != 0 when INVERT), kNone when the row has none.  One
+// wave per row: a ballot per 64-position chunk gives the chunk's site mask
+// (lane c keeps chunk c's), prefix max / suffix min over the chunks give the
+// nearest site before and after each chunk, and each position combines them
+// with its own chunk's mask.  Reads 1 byte and writes 4 per voxel, both
+// coalesced.
+template <bool INVERT>
+__global__ __launch_bounds__(256) void k_edt_rows(const uint8_t *__restrict__ data, int64_t rows, int nx,
+                                                  uint32_t *__restrict__ out) {
+    constexpr int kNoSite = 1 << 30;
+    const int lane = threadIdx.x & 63;
+    const int nch = (nx + 63) / 64;  // <= 64
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < rows; r += wstride) {
+        const uint8_t *row = data + r * nx;
+        uint32_t *orow = out + r * nx;
+        uint32_t mlo = 0, mhi = 0;  // lane c: chunk c's site mask
+        for (int c = 0; c < nch; ++c) {
+            const int x = 64 * c + lane;
+            bool site = false;
+            if (x < nx) site = (row[x] == 0) != INVERT;
+            const uint64_t m = __ballot(site);
+            if (lane == c) {
+                mlo = (uint32_t)m;
+                mhi = (uint32_t)(m >> 32);
+            }
+        }
+        const uint64_t mine = ((uint64_t)mhi << 32) | mlo;
+        int last = lane < nch && mine != 0 ? 64 * lane + top_bit(mine) : -1;
+        int first = lane < nch && mine != 0 ? 64 * lane + low_bit(mine) : kNoSite;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int a = __shfl_up(last, d, 64);
+            const int b = __shfl_down(first, d, 64);
+            if (lane >= d) last = max(last, a);
+            if (lane + d < 64) first = min(first, b);
+        }
+        int before = __shfl_up(last, 1, 64);  // nearest site before chunk `lane`
+        int after = __shfl_down(first, 1, 64);
+        if (lane == 0) before = -1;
+        if (lane == 63) after = kNoSite;
+        for (int c = 0; c < nch; ++c) {
+            const uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)mhi, c) << 32) |
+                               (uint32_t)__builtin_amdgcn_readlane((int)mlo, c);
+            const int bc = __builtin_amdgcn_readlane(before, c), ac = __builtin_amdgcn_readlane(after, c);
+            const int x = 64 * c + lane;
+            const uint64_t below = m & bits_below(lane + 1), above = m & ~bits_below(lane);
+            const int l = below != 0 ? 64 * c + top_bit(below) : bc;
+            const int nx_site = above != 0 ? 64 * c + low_bit(above) : ac;
+            int d = kNoSite;
+            if (l >= 0) d = x - l;
+            if (nx_site < kNoSite) d = min(d, nx_site - x);
+            if (x < nx) orow[x] = d < kNoSite ? (uint32_t)(d * d) : kNone;
+        }
+    }
+}
+
 // general.hh:96-118 on the float field: (uint16)((f - vmin) / (vmax - vmin) *
 // 65535) with vmax starting at FLT_MIN (general.hh:78), then x cylinder.
-__global__ __launch_bounds__(256) void k_edt_to_u16(const float *__restrict__ f, int64_t n, int64_t layer,
-                                                    const uint8_t *__restrict__ cyl,
+// Rows of nx voxels, 4 per thread (one 16-byte load where nx % 4 == 0).
+__global__ __launch_bounds__(256) void k_edt_to_u16(const float *__restrict__ f, int64_t rows, int64_t ny,
+                                                    int64_t nx, const uint8_t *__restrict__ cyl,
                                                     const uint32_t *__restrict__ mm,
                                                     uint16_t *__restrict__ out) {
     const float vmin = ord2f_d(mm[0]);
     const float vmax = fmaxf(ord2f_d(mm[1]), FLT_MIN);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint16_t q = (uint16_t)((f[i] - vmin) / (vmax - vmin) * 65535.0f);
-        if (cyl != nullptr && cyl[i % layer] == 0) q = 0;
-        out[i] = q;
+    const float span = vmax - vmin;
+    const int64_t per_row = (nx + 1023) / 1024;
+    const int64_t items = rows * per_row;
+    const bool vec = nx % 4 == 0;
+    for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const int64_t r = it / per_row;
+        const int64_t x0 = (it % per_row) * 1024 + 4 * threadIdx.x;
+        if (x0 >= nx) continue;
+        const float *fr = f + r * nx;
+        const uint8_t *cr = cyl != nullptr ? cyl + (r % ny) * nx : nullptr;
+        uint16_t *orow = out + r * nx;
+        float v[4];
+        if (vec) {
+            const float4 q = *reinterpret_cast<const float4 *>(fr + x0);
+            v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] = x0 + k < nx ? fr[x0 + k] : 0.0f;
+        }
+        uint16_t o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            o[k] = (uint16_t)((v[k] - vmin) / span * 65535.0f);
+            if (cr != nullptr && x0 + k < nx && cr[x0 + k] == 0) o[k] = 0;
+        }
+        if (vec) {
+            *reinterpret_cast<uint2 *>(orow + x0) =
+                make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
+        } else {
+            for (int k = 0; k < 4 && x0 + k < nx; ++k) orow[x0 + k] = o[k];
+        }
     }
 }
 
@@ -422,27 +507,27 @@ size_t pass_smem(const EdtPass &P) {
     return (size_t)4 * P.lines * ((size_t)P.pitch + (size_t)(64 * L) + kMetaInts);
 }
 
-template <int L, int MODE>
+template <int L, bool LAST>
 hipError_t launch_l(const EdtPass &P, hipStream_t s) {
     const size_t smem = pass_smem(P);
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_edt_pass<L, MODE>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_edt_pass<L, LAST>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
     if (e != hipSuccess) return e;
     const int64_t blocks = ((P.n_inner + P.lines - 1) / P.lines) * P.n_outer;
-    hipLaunchKernelGGL((k_edt_pass<L, MODE>), dim3((unsigned)blocks), dim3(P.lines * 64), smem, s, P);
+    hipLaunchKernelGGL((k_edt_pass<L, LAST>), dim3((unsigned)blocks), dim3(P.lines * 64), smem, s, P);
     return hipGetLastError();
 }
 
-template <int MODE>
+template <bool LAST>
 hipError_t launch_mode(const EdtPass &P, hipStream_t s) {
     const int64_t need = (P.n + 63) / 64;
-    if (need <= 1) return launch_l<1, MODE>(P, s);
-    if (need <= 2) return launch_l<2, MODE>(P, s);
-    if (need <= 4) return launch_l<4, MODE>(P, s);
-    if (need <= 8) return launch_l<8, MODE>(P, s);
-    if (need <= 16) return launch_l<16, MODE>(P, s);
-    if (need <= 32) return launch_l<32, MODE>(P, s);
-    return launch_l<64, MODE>(P, s);
+    if (need <= 1) return launch_l<1, LAST>(P, s);
+    if (need <= 2) return launch_l<2, LAST>(P, s);
+    if (need <= 4) return launch_l<4, LAST>(P, s);
+    if (need <= 8) return launch_l<8, LAST>(P, s);
+    if (need <= 16) return launch_l<16, LAST>(P, s);
+    if (need <= 32) return launch_l<32, LAST>(P, s);
+    return launch_l<64, LAST>(P, s);
 }
 
 // Geometry of the pass along axis a (0 z, 1 y, 2 x) of an (nz, ny, nx) volume.
@@ -466,14 +551,20 @@ EdtPass pass_geom(int a, int64_t nz, int64_t ny, int64_t nx) {
     return P;
 }
 
-hipError_t run_pass(int mode, EdtPass P, hipStream_t s) {
+hipError_t run_pass(bool last, EdtPass P, hipStream_t s) {
     if (P.n_inner * P.n_outer == 0 || P.n == 0) return hipSuccess;
-    switch (mode) {
-        case 0: return launch_mode<0>(P, s);
-        case 1: return launch_mode<1>(P, s);
-        case 2: return launch_mode<2>(P, s);
-        default: return launch_mode<3>(P, s);
-    }
+    return last ? launch_mode<true>(P, s) : launch_mode<false>(P, s);
+}
+
+hipError_t run_rows(const uint8_t *data, bool invert, int64_t rows, int64_t nx, uint32_t *out, hipStream_t s) {
+    if (rows == 0 || nx == 0) return hipSuccess;
+    int64_t blocks = (rows + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    if (invert)
+        hipLaunchKernelGGL(k_edt_rows<true>, dim3((unsigned)blocks), dim3(256), 0, s, data, rows, (int)nx, out);
+    else
+        hipLaunchKernelGGL(k_edt_rows<false>, dim3((unsigned)blocks), dim3(256), 0, s, data, rows, (int)nx, out);
+    return hipGetLastError();
 }
 
 int64_t vol_words_bytes(int64_t n) { return (n * 4 + 255) / 256 * 256; }
@@ -505,17 +596,16 @@ int check_extents(const char *who, int64_t nz, int64_t ny, int64_t nx) {
 int device_edt_core(const uint8_t *data, int64_t nz, int64_t ny, int64_t nx, bool invert, uint32_t *a,
                     uint32_t *b, uint32_t *last_out, int last_mode, const uint8_t *cyl, uint32_t *mm,
                     hipStream_t s) {
-    EdtPass px = pass_geom(2, nz, ny, nx), py = pass_geom(1, nz, ny, nx), pz = pass_geom(0, nz, ny, nx);
-    px.in = data; px.out = a;
+    EdtPass py = pass_geom(1, nz, ny, nx), pz = pass_geom(0, nz, ny, nx);
     py.in = a; py.out = b;
     pz.in = b; pz.out = last_out; pz.cyl = cyl; pz.minmax = mm;
     if (last_mode == 3) {
         hipLaunchKernelGGL(k_edt_minmax_init, dim3(1), dim3(64), 0, s, mm);
         MXB_EHIP(hipGetLastError());
     }
-    MXB_EHIP(run_pass(invert ? 1 : 0, px, s));
-    MXB_EHIP(run_pass(2, py, s));
-    MXB_EHIP(run_pass(last_mode, pz, s));
+    MXB_EHIP(run_rows(data, invert, nz * ny, nx, a, s));
+    MXB_EHIP(run_pass(false, py, s));
+    MXB_EHIP(run_pass(last_mode == 3, pz, s));
     return OK;
 }
 
@@ -587,9 +677,9 @@ int mxb_device_edt_field(const uint8_t *dev_implant_solid, int64_t nz, int64_t n
     // 0950:41 negates the implant mask: the sites are the implant voxels
     if ((rc = device_edt_core(dev_implant_solid, nz, ny, nx, true, a, b, a, 3, dev_cylinder, mm, s)) != OK)
         return rc;
-    int64_t blocks = (n + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_edt_to_u16, dim3((unsigned)blocks), dim3(256), 0, s, (const float *)a, n, ny * nx,
+    int64_t blocks = nz * ny * ((nx + 1023) / 1024);
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(k_edt_to_u16, dim3((unsigned)blocks), dim3(256), 0, s, (const float *)a, nz * ny, ny, nx,
                        dev_cylinder, mm, dev_out);
     MXB_EHIP(hipGetLastError());
     return OK;
