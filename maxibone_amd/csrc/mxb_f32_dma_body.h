@@ -248,14 +248,14 @@ struct F32Dma2Shape {
     static constexpr int C = LdsShape<RT>::C;
     static constexpr int W = LdsShape<RT>::W;
     static constexpr int NSL = W + 1 + 2 * D;
-    static constexpr int NM = D + 1;
 };
 
 template <int RT, int D, bool MASK, bool ZSTAGE>
 struct F32Dma2Smem {
     using S = F32Dma2Shape<RT, D>;
     static constexpr int RING = S::NSL * kF32Slot * 4;
-    static constexpr int MRING = MASK ? S::NM * 256 : 0;  // 64 lines x one 32-bit mask word
+    static constexpr int NM4 = (D + 7) / 4;  // mask slots of 4 pairs: reused only after pair i+3's epilogue
+    static constexpr int MRING = MASK ? NM4 * 1024 : 0;  // 64 lines x 4 pairs' 32-bit mask words
     static constexpr int STAGE = (ZSTAGE ? 32 * kF32StagePitchZ : 64 * kF32StagePitchV) * 4;
     static constexpr int TAPS = (4 * S::C + 32) * 4;
     static constexpr int BYTES = RING + MRING + STAGE + TAPS;
@@ -267,8 +267,12 @@ __device__ __forceinline__ void f32_dma_body2(const float *__restrict__ in, cons
                                               int64_t seg_id, const TapSrc &taps,
                                               unsigned char *__restrict__ smem) {
     using S = F32Dma2Shape<RT, D>;
-    constexpr int C = S::C, W = S::W, NSL = S::NSL, NM = S::NM;
-    constexpr int G = MASK ? 3 : 2;
+    constexpr int C = S::C, W = S::W, NSL = S::NSL;
+    // loads issued in D - 1 groups: 2 data slots each, and (with (D - 1) % 4
+    // == 0) exactly (D - 1) / 4 mask loads (one per 4 pairs)
+    constexpr int GW = 2 * (D - 1) + (MASK ? (D - 1) / 4 : 0);
+    static_assert(!MASK || (D - 1) % 4 == 0, "a fixed count of mask loads per D - 1 groups");
+    constexpr int NM4 = F32Dma2Smem<RT, D, MASK, ZSTAGE>::NM4;
     constexpr int ST = ZSTAGE ? 2 : (OUTF ? 2 : 0) + (OUTU ? 1 : 0);
     static_assert(!ZSTAGE || (OUTF && !OUTU && !MASK), "line-contiguous epilogue writes floats only");
     static_assert(ST >= 1, "at least one output");
@@ -310,22 +314,23 @@ __device__ __forceinline__ void f32_dma_body2(const float *__restrict__ in, cons
                              : static_cast<const void *>(g_dma_zero);
         dma16<ZSTAGE>(ring_lds + (uint32_t)((slot % NSL) * kF32Slot * 4), src);
     };
-    // ---- mask bits (lanes < 16): the 32-bit word of the pair's 32 positions of
-    // line 16 wv + lane (segments start at multiples of 32 positions)
-    const int ml = 16 * wv + lane;
+    // ---- mask bits, four pairs per load: lane -> line 16 wv + lane / 4, pair
+    // pj + lane % 4 (the 32-bit word of its 32 positions; segments start at
+    // multiples of 32), so a slot holds [line][4 pairs]
+    const int ml = 16 * wv + (lane >> 2), mk = lane & 3;
     const bool ml_ok = MASK && line0 + ml < g.n_lane;
     const uint32_t *msrc = MASK ? e.bits.w + outer * e.bits.outer + (line0 + ml) * e.bits.lane : nullptr;
-    const uint32_t mring_lds = lds_addr(mring) + 64u * (uint32_t)wv;
+    const uint32_t mring_lds = lds_addr(mring) + 256u * (uint32_t)wv;
     auto issue_mask = [&](int pj) {
-        const int64_t ppos = a0 + 32 * (int64_t)pj;
+        const int64_t ppos = a0 + 32 * (int64_t)(pj + mk);
         const bool ok = ml_ok && ppos + 32 <= a1;
         const void *src = ok ? static_cast<const void *>(msrc + ppos / 32) : static_cast<const void *>(g_dma_zero);
-        if (lane < 16) dma4(mring_lds + (uint32_t)((pj % NM) * 256), src);
+        dma4(mring_lds + (uint32_t)(((pj >> 2) % NM4) * 1024), src);
     };
     auto issue_group = [&](int i) {
         issue_data(2 * i + W - 1);
         issue_data(2 * i + W);
-        if (MASK) issue_mask(i);
+        if (MASK && (i & 3) == 0) issue_mask(i);
     };
 
     for (int s = 0; s < W - 1; ++s) issue_data(s);
@@ -347,9 +352,9 @@ __device__ __forceinline__ void f32_dma_body2(const float *__restrict__ in, cons
 
     for (int j = 0; j < npairs; ++j) {
         if (j >= D)
-            vm_wait<(D - 1) * G + D * ST>();
+            vm_wait<GW + D * ST>();
         else
-            vm_wait<(D - 1) * G>();
+            vm_wait<GW>();
         __syncthreads();
         issue_group(j + D);
         float xs[C + 4];
@@ -398,7 +403,8 @@ __device__ __forceinline__ void f32_dma_body2(const float *__restrict__ in, cons
             const int64_t off = outer * g.out_outer + (line0 + sl) * g.out_lane + p0;
             if (full) {
                 if (MASK) {
-                    const uint32_t m = *reinterpret_cast<const uint32_t *>(mring + (j % NM) * 256 + sl * 4) >> (8 * sq);
+                    const uint32_t m = *reinterpret_cast<const uint32_t *>(mring + ((j >> 2) % NM4) * 1024 + sl * 16 +
+                                                                           (j & 3) * 4) >> (8 * sq);
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         if ((m >> r) & 1u) v[0][r] = 1.0f;

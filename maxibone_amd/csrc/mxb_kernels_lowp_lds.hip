@@ -248,21 +248,23 @@ __global__ __launch_bounds__(256) void k_line_lowp_lds(const uint16_t *__restric
 
 // ---------------------------------------------------------------------------
 // Direct-load form: the same pass with the slots (and, for the illuminating
-// pass, the 64 lines' 32-bit mask words of each 32-position pair) moved
-// global -> LDS by global_load_lds_dword{x4,}, D slots ahead of use, with no
-// VGPR staging.
+// pass, the 64 lines' 32-bit mask words, four pairs at a time) moved global
+// -> LDS by global_load_lds_dword{x4,}, D slots ahead of use, with no VGPR
+// staging.
 //
 // Iteration j (tile pair j):
 //   A. wait until group j has landed (vm_wait), barrier;
 //   B. issue group j + D into the ring slots freed by pair j - 1;
 //   C. compute pair j from slots j .. j + W - 1;
 //   D. stage the outputs in LDS, barrier, one 16-byte store per thread.
-// Group i = data slot i + W - 1 (+ the mask tile of pair i).  The prologue
-// issues data slots 0 .. W - 2 and groups 0 .. D - 1.  After group j the
-// wave has issued groups j+1 .. j+D-1 and the stores of up to D pairs, so
-// the wait is vmcnt((D-1) G + min(j, D) S) with G loads per group and S = 1
-// store per pair; the min(j, D) term is dropped (a smaller count only waits
-// longer) for the first D pairs.  Loads outside the volume or past the
+// Group i = data slot i + W - 1 (+, when i % 4 == 0, the mask words of pairs
+// i .. i+3, one 4-byte load per lane).  The prologue issues data slots
+// 0 .. W - 2 and groups 0 .. D - 1.  After group j the wave has issued groups
+// j+1 .. j+D-1 (D - 1 data loads and, with (D - 1) % 4 == 0, exactly
+// (D - 1) / 4 mask loads) and the stores of up to D pairs, so the wait is
+// vmcnt(GW + min(j, D) S) with GW those loads and S = 1 store per pair; the
+// min(j, D) term is dropped (a smaller count only waits longer) for the
+// first D pairs.  Loads outside the volume or past the
 // segment read g_dma_zero and stores of lines beyond the volume go to
 // g_dma_sink, so every wave issues exactly these counts; the last pair of
 // a segment, when partial, is written element by element after its wait.
@@ -271,10 +273,10 @@ struct LowpDmaShape {
     static constexpr int C = LowpChunks<RT>::C;
     static constexpr int W = C + 1;    // slots a tile pair reads
     static constexpr int NSL = W + D;  // ring slots
-    static constexpr int NM = D + 1;   // mask tile slots
+    static constexpr int NM = (D + 7) / 4;  // mask slots of 4 pairs: reused only after pair i+3's epilogue
 };
 
-constexpr int kMaskSlotBytes = 64 * 4;  // 64 lines x one 32-bit word of mask bits
+constexpr int kMaskSlotBytes = 64 * 16;  // 64 lines x 4 pairs' 32-bit words of mask bits
 
 template <int RT, int D, bool BF, bool MASK, bool ZSTAGE>
 __global__ __launch_bounds__(256) void k_line_lowp_dma(const uint16_t *__restrict__ in,
@@ -286,7 +288,8 @@ __global__ __launch_bounds__(256) void k_line_lowp_dma(const uint16_t *__restric
     using L = Lowp<BF>;
     constexpr int C = S::C, W = S::W, NSL = S::NSL, NM = S::NM;
     constexpr int TW = LowpChunks<RT>::TW;
-    constexpr int G = MASK ? 2 : 1;  // loads per group
+    constexpr int GW = (D - 1) + (MASK ? (D - 1) / 4 : 0);  // loads in D - 1 groups
+    static_assert(!MASK || (D - 1) % 4 == 0, "a fixed count of mask loads per D - 1 groups");
     static_assert(!(MASK && ZSTAGE), "the illuminating pass writes positions contiguously");
     __shared__ __attribute__((aligned(16))) unsigned char ring[NSL * kSlotBytes];
     __shared__ __attribute__((aligned(16))) unsigned char mring[MASK ? NM * kMaskSlotBytes : 16];
@@ -329,21 +332,22 @@ __global__ __launch_bounds__(256) void k_line_lowp_dma(const uint16_t *__restric
                              : static_cast<const void *>(g_dma_zero);
         dma16<ZSTAGE>(ring_lds + (uint32_t)((slot % NSL) * kSlotBytes), src);
     };
-    // ---- mask bits (lanes < 16): the 32-bit word of the pair's 32 positions of
-    // line 16 wv + lane (segments start at multiples of 32 positions)
-    const int ml = 16 * wv + lane;
+    // ---- mask bits: lane -> line 16 wv + lane / 4, pair pj + lane % 4 (the
+    // 32-bit word of its 32 positions; segments start at multiples of 32), so
+    // a slot holds [line][4 pairs]
+    const int ml = 16 * wv + (lane >> 2), mk = lane & 3;
     const bool ml_ok = MASK && line0 + ml < g.n_lane;
     const uint32_t *msrc = MASK ? bits.w + outer * bits.outer + (line0 + ml) * bits.lane : nullptr;
-    const uint32_t mring_lds = lds_addr(mring) + 64u * (uint32_t)wv;
+    const uint32_t mring_lds = lds_addr(mring) + 256u * (uint32_t)wv;
     auto issue_mask = [&](int pj) {
-        const int64_t ppos = a0 + 32 * (int64_t)pj;
+        const int64_t ppos = a0 + 32 * (int64_t)(pj + mk);
         const bool ok = ml_ok && ppos + 32 <= a1;
         const void *src = ok ? static_cast<const void *>(msrc + ppos / 32) : static_cast<const void *>(g_dma_zero);
-        if (lane < 16) dma4(mring_lds + (uint32_t)((pj % NM) * kMaskSlotBytes), src);
+        dma4(mring_lds + (uint32_t)(((pj >> 2) % NM) * kMaskSlotBytes), src);
     };
     auto issue_group = [&](int i) {
         issue_data(i + W - 1);
-        if (MASK) issue_mask(i);
+        if (MASK && (i & 3) == 0) issue_mask(i);
     };
 
     for (int s = 0; s < W - 1; ++s) issue_data(s);
@@ -374,9 +378,9 @@ __global__ __launch_bounds__(256) void k_line_lowp_dma(const uint16_t *__restric
     for (int j = 0; j < npairs; ++j) {
         // A. group j has landed in every wave
         if (j >= D)
-            vm_wait<(D - 1) * G + D>();
+            vm_wait<GW + D>();
         else
-            vm_wait<(D - 1) * G>();
+            vm_wait<GW>();
         __syncthreads();
         // B. prefetch group j + D
         issue_group(j + D);
@@ -435,8 +439,8 @@ __global__ __launch_bounds__(256) void k_line_lowp_dma(const uint16_t *__restric
             const int64_t off = outer * g.out_outer + (line0 + sl) * g.out_lane + p0;
             if (full) {
                 if (MASK) {
-                    const uint32_t m =
-                        *reinterpret_cast<const uint32_t *>(mring + (j % NM) * kMaskSlotBytes + sl * 4) >> (8 * sq);
+                    const uint32_t m = *reinterpret_cast<const uint32_t *>(mring + ((j >> 2) % NM) * kMaskSlotBytes +
+                                                                           sl * 16 + (j & 3) * 4) >> (8 * sq);
                     const uint32_t one = L::from_f32(1.0f);
                     uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
