@@ -149,7 +149,7 @@ struct VolLayout {
     int64_t nz, ny, nx, px, py;
     int64_t st_off;    // floats, into each state buffer
     int64_t mask_off;  // bytes, into the pitched masks
-    int64_t bits_off;  // words, into the pitched mask bits ((nz, py) rows of words_per_row(px))
+    int64_t bits_off;  // words, into the pitched mask bits (nz x words_per_row(px) x py)
     int R, bucket;
 };
 
@@ -273,7 +273,7 @@ static int build_tables(mxb_batch_plan *P, int seg) {
                 set_segments(D.g, seg);
                 const int64_t wpr = mask_words_per_row(L.px);
                 D.e = p == 2 ? Epilogue{P->masks + L.mask_off, outs[p], nullptr,
-                                        MaskBits{P->bits + L.bits_off, wpr, L.py * wpr}}
+                                        MaskBits{P->bits + L.bits_off, 1, wpr * L.py, L.py}}
                              : Epilogue{nullptr, outs[p], nullptr, MaskBits{}};
                 D.bucket = L.bucket;
                 D.taps_off = 0;
@@ -572,8 +572,8 @@ int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, 
             set_error(std::string("k_pitch_mask: ") + hipGetErrorString(e));
             return fail(EDEVICE);
         }
-        if ((e = launch_pack_mask_bits(P->masks + L.mask_off, L.nz * L.py, L.px, L.px, P->bits + L.bits_off,
-                                       mask_words_per_row(L.px), P->stream)) != hipSuccess) {
+        if ((e = launch_pack_mask_bits(P->masks + L.mask_off, L.nz, L.py, L.px, L.px, L.py * L.px,
+                                       P->bits + L.bits_off, P->stream)) != hipSuccess) {
             set_error(std::string("k_pack_mask_bits: ") + hipGetErrorString(e));
             return fail(EDEVICE);
         }

@@ -142,8 +142,8 @@ int64_t state_bytes(int precision, int64_t nz, int64_t ny, int64_t nx) {
 static hipError_t pack_bits(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t nx, uint32_t *w,
                             MaskBits &bits, hipStream_t s) {
     const int64_t wpr = mask_words_per_row(nx);
-    bits = MaskBits{w, wpr, ny * wpr};
-    return launch_pack_mask_bits(d_mask, nz * ny, nx, nx, w, wpr, s);
+    bits = MaskBits{w, 1, wpr * ny, ny};
+    return launch_pack_mask_bits(d_mask, nz, ny, nx, nx, ny * nx, w, s);
 }
 
 // ---------------------------------------------------------------- driver

@@ -314,17 +314,19 @@ __device__ __forceinline__ void f32_dma_body2(const float *__restrict__ in, cons
                              : static_cast<const void *>(g_dma_zero);
         dma16<ZSTAGE>(ring_lds + (uint32_t)((slot % NSL) * kF32Slot * 4), src);
     };
-    // ---- mask bits, four pairs per load: lane -> line 16 wv + lane / 4, pair
-    // pj + lane % 4 (the 32-bit word of its 32 positions; segments start at
-    // multiples of 32), so a slot holds [line][4 pairs]
-    const int ml = 16 * wv + (lane >> 2), mk = lane & 3;
+    // ---- mask bits, four pairs per load: lane -> pair pj + lane / 16, line
+    // 16 wv + lane % 16 (the 32-bit word of its 32 positions; segments start
+    // at multiples of 32), so a slot holds [wave][4 pairs][16 lines] and each
+    // load reads 64 contiguous bytes per pair
+    const int ml = 16 * wv + (lane & 15), mk = lane >> 4;
     const bool ml_ok = MASK && line0 + ml < g.n_lane;
     const uint32_t *msrc = MASK ? e.bits.w + outer * e.bits.outer + (line0 + ml) * e.bits.lane : nullptr;
     const uint32_t mring_lds = lds_addr(mring) + 256u * (uint32_t)wv;
     auto issue_mask = [&](int pj) {
         const int64_t ppos = a0 + 32 * (int64_t)(pj + mk);
         const bool ok = ml_ok && ppos + 32 <= a1;
-        const void *src = ok ? static_cast<const void *>(msrc + ppos / 32) : static_cast<const void *>(g_dma_zero);
+        const void *src = ok ? static_cast<const void *>(msrc + (ppos / 32) * e.bits.word)
+                             : static_cast<const void *>(g_dma_zero);
         dma4(mring_lds + (uint32_t)(((pj >> 2) % NM4) * 1024), src);
     };
     auto issue_group = [&](int i) {
@@ -403,8 +405,9 @@ __device__ __forceinline__ void f32_dma_body2(const float *__restrict__ in, cons
             const int64_t off = outer * g.out_outer + (line0 + sl) * g.out_lane + p0;
             if (full) {
                 if (MASK) {
-                    const uint32_t m = *reinterpret_cast<const uint32_t *>(mring + ((j >> 2) % NM4) * 1024 + sl * 16 +
-                                                                           (j & 3) * 4) >> (8 * sq);
+                    const uint32_t m = *reinterpret_cast<const uint32_t *>(
+                        mring + ((j >> 2) % NM4) * 1024 + (sl >> 4) * 256 + (j & 3) * 64 + (sl & 15) * 4) >>
+                        (8 * sq);
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         if ((m >> r) & 1u) v[0][r] = 1.0f;

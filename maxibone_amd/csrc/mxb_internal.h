@@ -33,20 +33,23 @@ struct LineGeom {
     int32_t stage_out;               // 1: out_lane == 1, rows 16-byte aligned: stage tiles in LDS
 };
 
-// The mask packed to bits along x: bit (x mod 32) of word
-// [(z ny + y) * words_per_row(nx) + x / 32] is (mask[z][y][x] != 0).  The
+// The mask packed to bits along x, words interleaved across y: bit (x mod 32)
+// of word [(z * words_per_row(nx) + x / 32) * ny + y] is (mask[z][y][x] != 0),
+// so one 32-position word of consecutive lines (y) is contiguous.  The
 // illuminating pass's direct-load forms read it (1/8 byte per voxel instead of
 // 1 byte); every other form reads the byte mask.
 struct MaskBits {
-    const uint32_t *w = nullptr;  // nullptr: not available
-    int64_t lane = 0, outer = 0;  // word strides of the pass's lines and outer index
+    const uint32_t *w = nullptr;            // nullptr: not available
+    int64_t lane = 0, outer = 0, word = 0;  // word strides: the pass's lines, outer index, 32 positions
 };
 inline int64_t mask_words_per_row(int64_t nx) { return (nx + 31) / 32; }
 inline int64_t mask_bits_bytes(int64_t nz, int64_t ny, int64_t nx) {
     return nz * ny * mask_words_per_row(nx) * 4;
 }
-hipError_t launch_pack_mask_bits(const uint8_t *mask, int64_t rows, int64_t nx, int64_t mask_row,
-                                 uint32_t *bits, int64_t bits_row, hipStream_t s);
+// Packs nz planes of ny rows of nx mask bytes (row stride mask_row, plane
+// stride mask_plane) into the layout above.
+hipError_t launch_pack_mask_bits(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx, int64_t mask_row,
+                                 int64_t mask_plane, uint32_t *bits, hipStream_t s);
 
 // Epilogue of the last pass of a step: illuminate from the mask, write the
 // float state and/or its uint16 quantisation.

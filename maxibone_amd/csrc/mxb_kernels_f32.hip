@@ -350,35 +350,35 @@ hipError_t launch_quantise(const float *u, uint16_t *q, float *f, int64_t n, hip
     return hipGetLastError();
 }
 
-// Pack a mask (rows of nx bytes, row stride mask_row) into bits, one 32-bit
-// word per 32 positions of a row (row stride bits_row words); positions past
-// nx are 0.  One wave per 64 positions of a row: each lane tests one byte and
-// a ballot collects the two words.
-__global__ __launch_bounds__(256) void k_pack_mask_bits(const uint8_t *__restrict__ m, int64_t rows,
-                                                        int64_t nx, int64_t mask_row,
-                                                        uint32_t *__restrict__ bits, int64_t bits_row) {
+// Pack a mask into bits, words interleaved across rows (the MaskBits layout):
+// one wave per 64 positions of a row; each lane tests one byte and a ballot
+// collects the two words.
+__global__ __launch_bounds__(256) void k_pack_mask_bits(const uint8_t *__restrict__ m, int64_t nz, int64_t ny,
+                                                        int64_t nx, int64_t mask_row, int64_t mask_plane,
+                                                        uint32_t *__restrict__ bits) {
     const int64_t wpr = (nx + 31) / 32, cpr = (nx + 63) / 64;
-    const int64_t total = rows * cpr;
+    const int64_t total = nz * ny * cpr;
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < total; i += nwaves) {
-        const int64_t row = i / cpr, c = i % cpr;
+        const int64_t c = i % cpr, row = i / cpr;
+        const int64_t y = row % ny, z = row / ny;
         const int64_t x = 64 * c + lane;
-        const bool on = x < nx && m[row * mask_row + x] != 0;
+        const bool on = x < nx && m[z * mask_plane + y * mask_row + x] != 0;
         const unsigned long long b = __ballot(on);
         const int64_t w = 2 * c + lane;
-        if (lane < 2 && w < wpr) bits[row * bits_row + w] = (uint32_t)(b >> (32 * lane));
+        if (lane < 2 && w < wpr) bits[(z * wpr + w) * ny + y] = (uint32_t)(b >> (32 * lane));
     }
 }
 
-hipError_t launch_pack_mask_bits(const uint8_t *mask, int64_t rows, int64_t nx, int64_t mask_row,
-                                 uint32_t *bits, int64_t bits_row, hipStream_t s) {
-    const int64_t chunks = rows * ((nx + 63) / 64);  // one wave each
+hipError_t launch_pack_mask_bits(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx, int64_t mask_row,
+                                 int64_t mask_plane, uint32_t *bits, hipStream_t s) {
+    const int64_t chunks = nz * ny * ((nx + 63) / 64);  // one wave each
     if (chunks == 0) return hipSuccess;
     int64_t blocks = (chunks + 3) / 4;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_pack_mask_bits, dim3((unsigned)blocks), dim3(256), 0, s, mask, rows, nx, mask_row,
-                       bits, bits_row);
+    hipLaunchKernelGGL(k_pack_mask_bits, dim3((unsigned)blocks), dim3(256), 0, s, mask, nz, ny, nx, mask_row,
+                       mask_plane, bits);
     return hipGetLastError();
 }
 
