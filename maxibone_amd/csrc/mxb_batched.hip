@@ -19,6 +19,13 @@
 // The grid size and the segment length are chosen by timing candidates on one
 // repetition of the actual batch (autotune); the choice is cached per batch
 // signature in the process (and in the file named by MXB_TUNE_FILE, if set).
+//
+// BF16 / FP16 plans run the 16-bit direct-load body (mxb_lowp_dma_body.h) in
+// the same way for repetitions 1 .. reps-1, with the 16-bit state in the first
+// half of each volume's pitched state, then widen it and run the last
+// repetition with the FP32 kernels, as the single-volume path does.  Their
+// radius buckets are the single-volume 16-bit ones (the bucket fixes how taps
+// group into 32-position MFMA chunks), so the results are bit-identical to it.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -31,12 +38,22 @@
 
 #include "maxibone_diffusion.h"
 #include "mxb_f32_dma_body.h"
+#include "mxb_lowp_dma_body.h"
 
 namespace mxb {
 
 // Radius buckets of the persistent kernel (compiled tap capacities).
 static constexpr int kBucketRT[] = {8, 12, 16, 20, 24, 32, 40, 48, 64};
 static constexpr int kNumBuckets = sizeof(kBucketRT) / sizeof(kBucketRT[0]);
+// Prefetch depth of the 16-bit persistent kernel (tile pairs ahead; the
+// masked body needs D - 1 to be a multiple of 4).
+#ifndef MXB_LOWP_BATCH_DEPTH
+#define MXB_LOWP_BATCH_DEPTH 1
+#endif
+// Radius buckets of the 16-bit persistent kernel: those of the single-volume
+// 16-bit passes (launch_line_pass_lowp_lds).
+static constexpr int kLowpBucketRT[] = {4, 8, 12, 16, 24, 32, 40, 48, 64};
+static constexpr int kNumLowpBuckets = sizeof(kLowpBucketRT) / sizeof(kLowpBucketRT[0]);
 
 struct BatchPassDesc {
     const float *in;
@@ -104,6 +121,79 @@ static int smem_bytes_of(int bucket) {
 
 static int chunks_of(int bucket) { return (16 + 2 * kBucketRT[bucket] + 3) / 4; }
 
+// ---- 16-bit passes
+struct BatchLowpDesc {
+    const uint16_t *in;
+    uint16_t *out;
+    const uint8_t *mask;  // x-pass: pitched byte mask (partial tiles)
+    MaskBits bits;        // x-pass: pitched mask bits
+    LineGeom g;
+    int32_t bucket;       // index into kLowpBucketRT
+    int32_t taps_off;     // 2 RT + 1 high parts, then 2 RT + 1 low parts (uint16)
+};
+
+template <int FORM, bool BF, int RT>
+__device__ __forceinline__ void batch_lowp_body(const BatchLowpDesc &d, const BatchItem &it,
+                                                const uint16_t *__restrict__ taps, unsigned char *smem) {
+    lowp_dma_body<RT, MXB_LOWP_BATCH_DEPTH, BF, FORM == 2, FORM == 0>(
+        d.in, d.out, d.mask, d.bits, d.g, taps + d.taps_off, taps + d.taps_off + 2 * RT + 1, it.outer, it.lb,
+        it.seg, smem);
+}
+
+template <int FORM, bool BF>
+__global__ __launch_bounds__(256) void k_batched_lowp_pass(const BatchLowpDesc *__restrict__ descs,
+                                                           const BatchItem *__restrict__ items, int n_items,
+                                                           const uint16_t *__restrict__ taps) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char bsmem[];
+    for (int i = blockIdx.x; i < n_items; i += gridDim.x) {
+        const BatchItem it = items[i];
+        const BatchLowpDesc d = descs[it.vol];
+        switch (d.bucket) {
+            case 0: batch_lowp_body<FORM, BF, 4>(d, it, taps, bsmem); break;
+            case 1: batch_lowp_body<FORM, BF, 8>(d, it, taps, bsmem); break;
+            case 2: batch_lowp_body<FORM, BF, 12>(d, it, taps, bsmem); break;
+            case 3: batch_lowp_body<FORM, BF, 16>(d, it, taps, bsmem); break;
+            case 4: batch_lowp_body<FORM, BF, 24>(d, it, taps, bsmem); break;
+            case 5: batch_lowp_body<FORM, BF, 32>(d, it, taps, bsmem); break;
+            case 6: batch_lowp_body<FORM, BF, 40>(d, it, taps, bsmem); break;
+            case 7: batch_lowp_body<FORM, BF, 48>(d, it, taps, bsmem); break;
+            default: batch_lowp_body<FORM, BF, 64>(d, it, taps, bsmem); break;
+        }
+    }
+}
+
+template <int FORM, int RT>
+static constexpr int lowp_form_smem() {
+    return LowpDmaSmem<RT, MXB_LOWP_BATCH_DEPTH, FORM == 2, FORM == 0>::BYTES;
+}
+
+template <int FORM>
+static int lowp_smem_bytes_of(int bucket) {
+    switch (bucket) {
+        case 0: return lowp_form_smem<FORM, 4>();
+        case 1: return lowp_form_smem<FORM, 8>();
+        case 2: return lowp_form_smem<FORM, 12>();
+        case 3: return lowp_form_smem<FORM, 16>();
+        case 4: return lowp_form_smem<FORM, 24>();
+        case 5: return lowp_form_smem<FORM, 32>();
+        case 6: return lowp_form_smem<FORM, 40>();
+        case 7: return lowp_form_smem<FORM, 48>();
+        default: return lowp_form_smem<FORM, 64>();
+    }
+}
+
+// The centred 16-bit tap split of split_taps<RT> (mxb_lowp.h) for a runtime RT.
+static void split_taps_into(const float *k, int R, int RT, bool bf, uint16_t *hi, uint16_t *lo) {
+    for (int i = 0; i < 2 * RT + 1; ++i) {
+        const int r = i - RT;
+        const float v = (r >= -R && r <= R) ? k[R + r] : 0.0f;
+        const uint16_t h = bf ? bf16_bits(v) : f16_bits(v);
+        const float rest = v - (bf ? bf16_value(h) : f16_value(h));
+        hi[i] = h;
+        lo[i] = bf ? bf16_bits(rest) : f16_bits(rest);
+    }
+}
+
 // Pitched-layout initialisation and final unpadding.
 __global__ __launch_bounds__(256) void k_unpad_out(const float *__restrict__ st, int64_t nz,
                                                    int64_t ny, int64_t nx, int64_t py, int64_t px,
@@ -151,6 +241,7 @@ struct VolLayout {
     int64_t mask_off;  // bytes, into the pitched masks
     int64_t bits_off;  // words, into the pitched mask bits (nz x words_per_row(px) x py)
     int R, bucket;
+    int lbucket;       // 16-bit radius bucket
 };
 
 struct TuneChoice {
@@ -202,6 +293,10 @@ struct mxb_batch_plan {
     float *st[2] = {nullptr, nullptr};
     float *taps = nullptr;
     mxb::BatchPassDesc *descs = nullptr;  // [6][n]: (parity, pass)
+    mxb::BatchLowpDesc *ldescs = nullptr; // [6][n]: (parity, pass), 16-bit plans
+    uint16_t *ltaps = nullptr;            // 16-bit tap table
+    bool lowp = false;                    // BF16 / FP16 plan with the persistent form
+    int lsmem_bytes[3] = {0, 0, 0};
     mxb::BatchItem *items = nullptr;      // z | y | x lists
     int n_items[3] = {0, 0, 0};
     int64_t item_off[3] = {0, 0, 0};
@@ -294,6 +389,37 @@ static int build_tables(mxb_batch_plan *P, int seg) {
     for (auto &D : d) D.taps_off = 0;
     for (int k = 0; k < 6; ++k)
         for (int v = 0; v < n; ++v) d[(size_t)k * n + v].taps_off = toff[v];
+    // 16-bit descriptors (same geometry and items) and split taps
+    std::vector<BatchLowpDesc> ld;
+    std::vector<uint16_t> ltable;
+    if (P->lowp) {
+        ld.resize(6 * (size_t)n);
+        const bool bf = P->precision == MXB_BF16;
+        for (int v = 0; v < n; ++v) {
+            const VolLayout &L = P->lay[v];
+            const int RT = kLowpBucketRT[L.lbucket];
+            const int32_t off = (int32_t)ltable.size();
+            ltable.resize(ltable.size() + 2 * (2 * RT + 1));
+            split_taps_into(P->host_taps[v].data(), L.R, RT, bf, ltable.data() + off, ltable.data() + off + 2 * RT + 1);
+            for (int parity = 0; parity < 2; ++parity) {
+                uint16_t *cur = reinterpret_cast<uint16_t *>(P->st[parity] + L.st_off);
+                uint16_t *tmp = reinterpret_cast<uint16_t *>(P->st[1 - parity] + L.st_off);
+                const uint16_t *ins[3] = {cur, tmp, cur};
+                uint16_t *outs[3] = {tmp, cur, tmp};
+                for (int p = 0; p < 3; ++p) {
+                    const BatchPassDesc &F = d[(size_t)(parity * 3 + p) * n + v];
+                    BatchLowpDesc &D = ld[(size_t)(parity * 3 + p) * n + v];
+                    D.in = ins[p];
+                    D.out = outs[p];
+                    D.mask = F.e.mask;
+                    D.bits = F.e.bits;
+                    D.g = F.g;
+                    D.bucket = L.lbucket;
+                    D.taps_off = off;
+                }
+            }
+        }
+    }
     for (int p = 0; p < 3; ++p) {
         std::vector<std::pair<int64_t, BatchItem>> tmp;
         for (int v = 0; v < n; ++v) {
@@ -313,7 +439,7 @@ static int build_tables(mxb_batch_plan *P, int seg) {
         for (auto &t : tmp) lists[p].push_back(t.second);
     }
     const int total = (int)(lists[0].size() + lists[1].size() + lists[2].size());
-    if (total > P->item_cap || table.size() > (size_t)(2 * 65 * n + 16)) {
+    if (total > P->item_cap || table.size() > (size_t)(2 * 65 * n + 16) || ltable.size() > (size_t)(4 * 65 * n + 16)) {
         set_error("batch work list exceeds its reserved space");
         return EDEVICE;
     }
@@ -327,6 +453,12 @@ static int build_tables(mxb_batch_plan *P, int seg) {
     }
     MXB_BHIP(hipMemcpyAsync(P->descs, d.data(), d.size() * sizeof(BatchPassDesc), hipMemcpyHostToDevice, P->stream));
     MXB_BHIP(hipMemcpyAsync(P->taps, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice, P->stream));
+    if (P->lowp) {
+        MXB_BHIP(hipMemcpyAsync(P->ldescs, ld.data(), ld.size() * sizeof(BatchLowpDesc), hipMemcpyHostToDevice,
+                                P->stream));
+        MXB_BHIP(hipMemcpyAsync(P->ltaps, ltable.data(), ltable.size() * sizeof(uint16_t), hipMemcpyHostToDevice,
+                                P->stream));
+    }
     MXB_BHIP(hipStreamSynchronize(P->stream));
     return OK;
 }
@@ -346,8 +478,61 @@ static hipError_t launch_pass(mxb_batch_plan *P, int parity, int p, int grid) {
     return hipGetLastError();
 }
 
+static hipError_t launch_lowp_pass(mxb_batch_plan *P, int parity, int p, int grid) {
+    const int n_items = P->n_items[p];
+    if (n_items == 0) return hipSuccess;
+    const unsigned g = (unsigned)std::min(grid, n_items);
+    const BatchLowpDesc *d = P->ldescs + (size_t)(parity * 3 + p) * P->n;
+    const BatchItem *it = P->items + P->item_off[p];
+    const size_t sm = (size_t)P->lsmem_bytes[p];
+    const bool bf = P->precision == MXB_BF16;
+#define MXB_LOWP_LAUNCH(F, B) \
+    hipLaunchKernelGGL((k_batched_lowp_pass<F, B>), dim3(g), dim3(256), sm, P->stream, d, it, n_items, P->ltaps)
+    if (p == 0) {
+        if (bf) MXB_LOWP_LAUNCH(0, true); else MXB_LOWP_LAUNCH(0, false);
+    } else if (p == 1) {
+        if (bf) MXB_LOWP_LAUNCH(1, true); else MXB_LOWP_LAUNCH(1, false);
+    } else {
+        if (bf) MXB_LOWP_LAUNCH(2, true); else MXB_LOWP_LAUNCH(2, false);
+    }
+#undef MXB_LOWP_LAUNCH
+    return hipGetLastError();
+}
+
 // Enqueue the whole persistent-form diffusion on P->stream.
 static int enqueue_persistent(mxb_batch_plan *P) {
+    if (P->lowp && P->reps > 1) {
+        // repetitions 1 .. reps-1 in 16-bit state (z: s0 -> s1, y: s1 -> s0,
+        // x: s0 -> s1, swap), widen, then one FP32 repetition
+        const bool bf = P->precision == MXB_BF16;
+        for (int v = 0; v < P->n; ++v) {
+            const VolLayout &L = P->lay[v];
+            MXB_BHIP(launch_init_lowp(P->masks + L.mask_off, reinterpret_cast<uint16_t *>(P->st[0] + L.st_off),
+                                      L.nz * L.py * L.px, bf, P->stream));
+        }
+        int parity = 0;
+        for (int64_t t = 0; t < P->reps - 1; ++t) {
+            for (int p = 0; p < 3; ++p) MXB_BHIP(launch_lowp_pass(P, parity, p, P->tune.grid));
+            parity ^= 1;
+        }
+        for (int v = 0; v < P->n; ++v) {
+            const VolLayout &L = P->lay[v];
+            MXB_BHIP(launch_lowp_to_f32(reinterpret_cast<const uint16_t *>(P->st[parity] + L.st_off),
+                                        P->st[1 - parity] + L.st_off, L.nz * L.py * L.px, bf, P->stream));
+        }
+        parity ^= 1;
+        for (int p = 0; p < 3; ++p) MXB_BHIP(launch_pass(P, parity, p, P->tune.grid));
+        parity ^= 1;
+        for (int v = 0; v < P->n; ++v) {
+            const VolLayout &L = P->lay[v];
+            const mxb_device_volume &V = P->vols[v];
+            const int64_t n = L.nz * L.ny * L.nx;
+            hipLaunchKernelGGL(k_unpad_out, dim3(blocks_for(n)), dim3(256), 0, P->stream, P->st[parity] + L.st_off,
+                               L.nz, L.ny, L.nx, L.py, L.px, V.dev_out_u16, V.dev_out_f32);
+            MXB_BHIP(hipGetLastError());
+        }
+        return OK;
+    }
     for (int v = 0; v < P->n; ++v) {
         const VolLayout &L = P->lay[v];
         const int64_t n = L.nz * L.py * L.px;
@@ -376,9 +561,11 @@ static int time_one_rep(mxb_batch_plan *P, int grid, double *ms) {
     hipEvent_t a, b;
     MXB_BHIP(hipEventCreate(&a));
     MXB_BHIP(hipEventCreate(&b));
-    for (int p = 0; p < 3; ++p) MXB_BHIP(launch_pass(P, 0, p, grid));  // warm
+    // 16-bit plans spend all but one repetition in the 16-bit passes
+    auto pass = [&](int p) { return P->lowp ? launch_lowp_pass(P, 0, p, grid) : launch_pass(P, 0, p, grid); };
+    for (int p = 0; p < 3; ++p) MXB_BHIP(pass(p));  // warm
     MXB_BHIP(hipEventRecord(a, P->stream));
-    for (int p = 0; p < 3; ++p) MXB_BHIP(launch_pass(P, 0, p, grid));
+    for (int p = 0; p < 3; ++p) MXB_BHIP(pass(p));
     MXB_BHIP(hipEventRecord(b, P->stream));
     MXB_BHIP(hipEventSynchronize(b));
     float f = 0;
@@ -401,9 +588,14 @@ static int autotune(mxb_batch_plan *P, int cus) {
     for (int v = 0; v < P->n; ++v) {
         const VolLayout &L = P->lay[v];
         const int64_t n = L.nz * L.py * L.px;
-        hipLaunchKernelGGL(k_init_pitched, dim3(blocks_for(n)), dim3(256), 0, P->stream, P->masks + L.mask_off,
-                           P->st[0] + L.st_off, n);
-        MXB_BHIP(hipGetLastError());
+        if (P->lowp) {
+            MXB_BHIP(launch_init_lowp(P->masks + L.mask_off, reinterpret_cast<uint16_t *>(P->st[0] + L.st_off), n,
+                                      P->precision == MXB_BF16, P->stream));
+        } else {
+            hipLaunchKernelGGL(k_init_pitched, dim3(blocks_for(n)), dim3(256), 0, P->stream,
+                               P->masks + L.mask_off, P->st[0] + L.st_off, n);
+            MXB_BHIP(hipGetLastError());
+        }
     }
     TuneChoice best;
     best.ms = 1e30;
@@ -479,8 +671,9 @@ int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, 
         P->own_stream = true;
     }
     P->vols.assign(vols, vols + n);
-    // the persistent form serves FP32 (MFMA) volumes with R <= 64
-    P->persistent = precision == MXB_FP32;
+    // the persistent form serves FP32 (MFMA) and BF16 / FP16 volumes with R <= 64
+    P->lowp = precision == MXB_BF16 || precision == MXB_FP16;
+    P->persistent = precision == MXB_FP32 || P->lowp;
     int64_t st_total = 0, mask_total = 0, bits_total = 0;
     std::string key = "b" + std::to_string(n) + "p" + std::to_string(precision);
     for (int v = 0; v < n; ++v) {
@@ -500,6 +693,9 @@ int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, 
         L.bucket = kNumBuckets - 1;
         for (int b = 0; b < kNumBuckets; ++b)
             if (L.R <= kBucketRT[b]) { L.bucket = b; break; }
+        L.lbucket = kNumLowpBuckets - 1;
+        for (int b = 0; b < kNumLowpBuckets; ++b)
+            if (L.R <= kLowpBucketRT[b]) { L.lbucket = b; break; }
         L.st_off = st_total;
         // the per-volume form runs mxb_device_diffusion on these buffers
         const int64_t sb = mxb_state_bytes(precision, V.nz, V.ny, V.nx);
@@ -539,6 +735,8 @@ int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, 
     const size_t o_s1 = carve((size_t)st_total * 4);
     const size_t o_taps = carve((size_t)taps_cap * 4);
     const size_t o_desc = carve(6 * (size_t)n * sizeof(BatchPassDesc));
+    const size_t o_ldesc = carve(P->lowp ? 6 * (size_t)n * sizeof(BatchLowpDesc) : 0);
+    const size_t o_ltaps = carve(P->lowp ? (size_t)(4 * 65 * (int64_t)n + 16) * sizeof(uint16_t) : 0);
     const size_t o_items = carve((size_t)P->item_cap * sizeof(BatchItem));
     hipError_t e = hipMalloc(&P->ws, off);
     if (e != hipSuccess) {
@@ -553,6 +751,10 @@ int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, 
     P->st[1] = (float *)(base + o_s1);
     P->taps = (float *)(base + o_taps);
     P->descs = (BatchPassDesc *)(base + o_desc);
+    if (P->lowp) {
+        P->ldescs = (BatchLowpDesc *)(base + o_ldesc);
+        P->ltaps = (uint16_t *)(base + o_ltaps);
+    }
     P->items = (BatchItem *)(base + o_items);
     int rc = OK;
     auto fail = [&](int code) {
@@ -583,6 +785,11 @@ int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, 
             P->smem_bytes[0] = std::max(P->smem_bytes[0], smem_bytes_of<0>(L.bucket));
             P->smem_bytes[1] = std::max(P->smem_bytes[1], smem_bytes_of<1>(L.bucket));
             P->smem_bytes[2] = std::max(P->smem_bytes[2], smem_bytes_of<2>(L.bucket));
+            if (P->lowp) {
+                P->lsmem_bytes[0] = std::max(P->lsmem_bytes[0], lowp_smem_bytes_of<0>(L.lbucket));
+                P->lsmem_bytes[1] = std::max(P->lsmem_bytes[1], lowp_smem_bytes_of<1>(L.lbucket));
+                P->lsmem_bytes[2] = std::max(P->lsmem_bytes[2], lowp_smem_bytes_of<2>(L.lbucket));
+            }
         }
         int dev = 0, cus = 256;
         if (hipGetDevice(&dev) == hipSuccess) {

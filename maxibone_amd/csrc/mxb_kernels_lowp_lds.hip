@@ -28,6 +28,7 @@
 // distinct 32-byte bank groups.
 #include "mxb_dma.h"
 #include "mxb_lowp.h"
+#include "mxb_lowp_dma_body.h"
 
 namespace mxb {
 
@@ -38,12 +39,6 @@ struct LowpLdsShape {
     static constexpr int NSL = W + 1;    // ring slots
 };
 
-constexpr int kSlotBytes = 32 * 128;       // 32 positions x 64 lines x 2 bytes
-constexpr int kStagePitchZ = 144;          // bytes per position row (z-staged epilogue)
-constexpr int kStagePitchV = 80;           // bytes per line row (position-contiguous epilogue)
-constexpr int kStageBytesL = 64 * kStagePitchV;  // >= 32 * kStagePitchZ
-
-__device__ __forceinline__ int row_swizzle(int r) { return ((r >> 1) & 1) | ((r >> 2) & 2); }
 
 template <int RT, bool BF>
 __global__ __launch_bounds__(256) void k_line_lowp_lds(const uint16_t *__restrict__ in,
@@ -247,227 +242,21 @@ __global__ __launch_bounds__(256) void k_line_lowp_lds(const uint16_t *__restric
 
 
 // ---------------------------------------------------------------------------
-// Direct-load form: the same pass with the slots (and, for the illuminating
-// pass, the 64 lines' 32-bit mask words, four pairs at a time) moved global
-// -> LDS by global_load_lds_dword{x4,}, D slots ahead of use, with no VGPR
-// staging.
-//
-// Iteration j (tile pair j):
-//   A. wait until group j has landed (vm_wait), barrier;
-//   B. issue group j + D into the ring slots freed by pair j - 1;
-//   C. compute pair j from slots j .. j + W - 1;
-//   D. stage the outputs in LDS, barrier, one 16-byte store per thread.
-// Group i = data slot i + W - 1 (+, when i % 4 == 0, the mask words of pairs
-// i .. i+3, one 4-byte load per lane).  The prologue issues data slots
-// 0 .. W - 2 and groups 0 .. D - 1.  After group j the wave has issued groups
-// j+1 .. j+D-1 (D - 1 data loads and, with (D - 1) % 4 == 0, exactly
-// (D - 1) / 4 mask loads) and the stores of up to D pairs, so the wait is
-// vmcnt(GW + min(j, D) S) with GW those loads and S = 1 store per pair; the
-// min(j, D) term is dropped (a smaller count only waits longer) for the
-// first D pairs.  Loads outside the volume or past the
-// segment read g_dma_zero and stores of lines beyond the volume go to
-// g_dma_sink, so every wave issues exactly these counts; the last pair of
-// a segment, when partial, is written element by element after its wait.
-template <int RT, int D>
-struct LowpDmaShape {
-    static constexpr int C = LowpChunks<RT>::C;
-    static constexpr int W = C + 1;    // slots a tile pair reads
-    static constexpr int NSL = W + D;  // ring slots
-    static constexpr int NM = (D + 7) / 4;  // mask slots of 4 pairs: reused only after pair i+3's epilogue
-};
-
-constexpr int kMaskSlotBytes = 64 * 16;  // 64 lines x 4 pairs' 32-bit words of mask bits
-
+// Direct-load form: the body is lowp_dma_body (mxb_lowp_dma_body.h), shared
+// with the batched persistent kernel; one block per (segment, 64 lines, outer).
 template <int RT, int D, bool BF, bool MASK, bool ZSTAGE>
 __global__ __launch_bounds__(256) void k_line_lowp_dma(const uint16_t *__restrict__ in,
                                                        uint16_t *__restrict__ out,
                                                        const uint8_t *__restrict__ mask,
                                                        MaskBits bits, LineGeom g,
                                                        TapsL<2 * RT + 1> taps) {
-    using S = LowpDmaShape<RT, D>;
-    using L = Lowp<BF>;
-    constexpr int C = S::C, W = S::W, NSL = S::NSL, NM = S::NM;
-    constexpr int TW = LowpChunks<RT>::TW;
-    constexpr int GW = (D - 1) + (MASK ? (D - 1) / 4 : 0);  // loads in D - 1 groups
-    static_assert(!MASK || (D - 1) % 4 == 0, "a fixed count of mask loads per D - 1 groups");
-    static_assert(!(MASK && ZSTAGE), "the illuminating pass writes positions contiguously");
-    __shared__ __attribute__((aligned(16))) unsigned char ring[NSL * kSlotBytes];
-    __shared__ __attribute__((aligned(16))) unsigned char mring[MASK ? NM * kMaskSlotBytes : 16];
-    __shared__ __attribute__((aligned(16))) unsigned char stage[kStageBytesL];
-    __shared__ uint16_t thi[TW], tlo[TW];
-
+    __shared__ __attribute__((aligned(16))) unsigned char smem[LowpDmaSmem<RT, D, MASK, ZSTAGE>::BYTES];
     int64_t bid = blockIdx.x;
     const int64_t seg_id = bid % g.nseg;
     bid /= g.nseg;
     const int64_t nlb = (g.n_lane + 63) >> 6;
-    const int64_t lb = bid % nlb;
-    const int64_t outer = bid / nlb;
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int col = lane & 15, kg = lane >> 4;
-    const int64_t line0 = lb * 64;
-    const int64_t a0 = seg_id * g.seg;
-    const int64_t a1 = a0 + g.seg < g.n_ax ? a0 + g.seg : g.n_ax;
-    const int64_t n_ax = g.n_ax;
-    const int npairs = (int)((a1 - a0 + 31) / 32);
-    const int nslots = npairs + W - 1;  // slots the segment reads
-
-    lowp_fill_taps<RT>(thi, tlo, taps);
-
-    // ---- data loads: lane -> row dr of the slot, 8 lines from dl (swizzled quads)
-    const int dr = 8 * wv + (lane >> 3);
-    const int pp = lane & 7;
-    const int dl = 16 * ((pp >> 1) ^ row_swizzle(dr)) + 8 * (pp & 1);
-    const bool dl_ok = line0 + dl < g.n_lane;  // n_lane is a multiple of 8
-    const char *dsrc = reinterpret_cast<const char *>(in + outer * g.in_outer + line0 + dl);
-    const int64_t in_ax_b = g.in_ax * 2;
-    const int64_t pos_first = a0 - RT;  // position of slot 0, row 0
-    const uint32_t ring_lds = lds_addr(ring) + 1024u * (uint32_t)wv;
-    auto issue_data = [&](int slot) {
-        const int64_t p = pos_first + 32 * (int64_t)slot + dr;
-        const bool ok = dl_ok && slot < nslots && p >= 0 && p < n_ax;
-        const void *src = ok ? static_cast<const void *>(dsrc + p * in_ax_b)
-                             : static_cast<const void *>(g_dma_zero);
-        dma16<ZSTAGE>(ring_lds + (uint32_t)((slot % NSL) * kSlotBytes), src);
-    };
-    // ---- mask bits: lane -> pair pj + lane / 16, line 16 wv + lane % 16 (the
-    // 32-bit word of its 32 positions; segments start at multiples of 32), so
-    // a slot holds [wave][4 pairs][16 lines] and each load reads 64
-    // contiguous bytes per pair
-    const int ml = 16 * wv + (lane & 15), mk = lane >> 4;
-    const bool ml_ok = MASK && line0 + ml < g.n_lane;
-    const uint32_t *msrc = MASK ? bits.w + outer * bits.outer + (line0 + ml) * bits.lane : nullptr;
-    const uint32_t mring_lds = lds_addr(mring) + 256u * (uint32_t)wv;
-    auto issue_mask = [&](int pj) {
-        const int64_t ppos = a0 + 32 * (int64_t)(pj + mk);
-        const bool ok = ml_ok && ppos + 32 <= a1;
-        const void *src = ok ? static_cast<const void *>(msrc + (ppos / 32) * bits.word)
-                             : static_cast<const void *>(g_dma_zero);
-        dma4(mring_lds + (uint32_t)(((pj >> 2) % NM) * kMaskSlotBytes), src);
-    };
-    auto issue_group = [&](int i) {
-        issue_data(i + W - 1);
-        if (MASK && (i & 3) == 0) issue_mask(i);
-    };
-
-    for (int s = 0; s < W - 1; ++s) issue_data(s);
-    for (int i = 0; i < D; ++i) issue_group(i);
-    __syncthreads();  // tap tables
-
-    v8s ahi[C], alo[C];
-    lowp_read_taps<RT>(thi, tlo, col, kg, ahi, alo);
-
-    int roff[2][2], radj[2][2];
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int rl = 16 * u + 8 * kg + (col >> 2) + 4 * h;
-            const int rr = rl & 31;
-            radj[u][h] = rl >> 5;
-            roff[u][h] = rr * 128 + ((wv ^ row_swizzle(rr)) << 5) + 8 * (col & 3);
-        }
-
-    // store-side constants
-    const int sl = tid >> 2, sq = tid & 3;         // position-contiguous: line, 8 positions
-    const int zr = tid >> 3, zq = tid & 7;         // line-contiguous: position, 8 lines
-    const bool sl_ok = line0 + sl < g.n_lane;
-    const bool zq_ok = line0 + 8 * zq < g.n_lane;
-    uint16_t *const sink = reinterpret_cast<uint16_t *>(g_dma_sink + lane);
-
-    for (int j = 0; j < npairs; ++j) {
-        // A. group j has landed in every wave
-        if (j >= D)
-            vm_wait<GW + D>();
-        else
-            vm_wait<GW>();
-        __syncthreads();
-        // B. prefetch group j + D
-        issue_group(j + D);
-        // C. tiles 2j, 2j+1 from slots j .. j + W - 1
-        const int sj = j % NSL;
-        v4f acc[2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) acc[u] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int c = 0; c < C; ++c)
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                v4s part[2];
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    int si = sj + c + radj[u][h];
-                    si = si >= NSL ? si - NSL : si;
-                    typedef __attribute__((address_space(3))) v4s lds_v4s;
-                    part[h] = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
-                        (lds_v4s *)(ring + si * kSlotBytes + roff[u][h]));
-                }
-                const v8s x = {part[0][0], part[0][1], part[0][2], part[0][3],
-                               part[1][0], part[1][1], part[1][2], part[1][3]};
-                acc[u] = L::mfma(ahi[c], x, acc[u]);
-                acc[u] = L::mfma(alo[c], x, acc[u]);
-            }
-        // D. epilogue
-        const int64_t ppos = a0 + 32 * (int64_t)j;
-        const bool full = ppos + 32 <= a1;  // uniform; only a segment's last pair is partial
-        if (ZSTAGE) {
-#pragma unroll
-            for (int u = 0; u < 2; ++u)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    *reinterpret_cast<uint16_t *>(stage + (16 * u + 4 * kg + r) * kStagePitchZ +
-                                                  (16 * wv + col) * 2) = L::from_f32(acc[u][r]);
-            __syncthreads();
-            const uint4 v = *reinterpret_cast<const uint4 *>(stage + zr * kStagePitchZ + 16 * zq);
-            const int64_t p = ppos + zr;
-            uint16_t *op = out + outer * g.out_outer + p * g.out_ax + line0 + 8 * zq;
-            if (full) {
-                store16<true>(zq_ok ? op : sink, v);
-            } else if (zq_ok && p < a1) {
-                *reinterpret_cast<uint4 *>(op) = v;
-            }
-        } else {
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const uint2 v = make_uint2(pack2(L::from_f32(acc[u][0]), L::from_f32(acc[u][1])),
-                                           pack2(L::from_f32(acc[u][2]), L::from_f32(acc[u][3])));
-                *reinterpret_cast<uint2 *>(stage + (16 * wv + col) * kStagePitchV + (16 * u + 4 * kg) * 2) = v;
-            }
-            __syncthreads();
-            uint4 v = *reinterpret_cast<const uint4 *>(stage + sl * kStagePitchV + 16 * sq);
-            const int64_t p0 = ppos + 8 * sq;
-            const int64_t off = outer * g.out_outer + (line0 + sl) * g.out_lane + p0;
-            if (full) {
-                if (MASK) {
-                    const uint32_t m = *reinterpret_cast<const uint32_t *>(
-                        mring + ((j >> 2) % NM) * kMaskSlotBytes + (sl >> 4) * 256 + (j & 3) * 64 + (sl & 15) * 4) >>
-                        (8 * sq);
-                    const uint32_t one = L::from_f32(1.0f);
-                    uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        if ((m >> e) & 1u) {
-                            const int sh = 16 * (e & 1);
-                            w[e >> 1] = (w[e >> 1] & ~(0xffffu << sh)) | (one << sh);
-                        }
-                    }
-                    v = make_uint4(w[0], w[1], w[2], w[3]);
-                }
-                store16<false>(sl_ok ? out + off : sink, v);
-            } else if (sl_ok && p0 < a1) {
-                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    if (p0 + e >= a1) break;
-                    uint16_t b = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
-                    if (MASK && mask[off + e] != 0) b = L::from_f32(1.0f);
-                    out[off + e] = b;
-                }
-            }
-        }
-    }
+    lowp_dma_body<RT, D, BF, MASK, ZSTAGE>(in, out, mask, bits, g, taps.hi, taps.lo, bid / nlb, bid % nlb,
+                                           seg_id, smem);
 }
 
 // ---------------------------------------------------------------- host side
@@ -500,9 +289,7 @@ static hipError_t launch_lowp_lds_rt(const uint16_t *in, uint16_t *out, const ui
     return hipGetLastError();
 }
 
-#ifndef MXB_LOWP_DMA_DEPTH
-#define MXB_LOWP_DMA_DEPTH 5
-#endif
+
 
 // Direct-load form: needs whole 8-line row pieces (n_lane a multiple of 8) and
 // 16-byte-aligned output rows written either line-contiguous or

@@ -187,11 +187,15 @@ def test_batched_host_entry(gpu, prec):
             assert np.array_equal(f, single)
             assert np.abs(f - u_ref).max() <= F32_TOL and _u16_close(q, q_ref)
         else:
+            # the persistent 16-bit passes equal the single-volume 16-bit path
+            q1, f1 = gpu.diffusion(m, k, 3, precision=prec, u16=True, f32=True)
+            assert np.array_equal(f, f1) and np.array_equal(q, q1)
             u64 = oracle.diffusion_f64(m, k.astype(np.float64), 3)
             assert np.abs(f - u64).max() <= LOWP_TOL
 
 
-def test_batched_plan_persistent_graph(gpu):
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_batched_plan_persistent_graph(gpu, prec):
     """The captured persistent graph equals per-volume launches, on every replay."""
     from maxibone_amd import hip
     masks, kernels = _batch_inputs(32, 9, 20, 150, 1.0, 10.0)
@@ -199,7 +203,7 @@ def test_batched_plan_persistent_graph(gpu):
     of = [hip.DeviceBuffer(4 * m.size) for m in masks]
     oq = [hip.DeviceBuffer(2 * m.size) for m in masks]
     plan = gpu.BatchPlan([b.ptr for b in dm], [m.shape for m in masks], kernels, 4,
-                         outs_u16=[b.ptr for b in oq], outs_f32=[b.ptr for b in of])
+                         outs_u16=[b.ptr for b in oq], outs_f32=[b.ptr for b in of], precision=prec)
     info = plan.info()
     _say("batch plan", info)
     assert info["persistent"] and info["grid"] > 0
@@ -217,8 +221,12 @@ def test_batched_plan_persistent_graph(gpu):
             assert np.array_equal(b.to_array(np.float32, m.shape), w)
             assert np.array_equal(bq.to_array(np.uint16, m.shape), wq)
     for m, k, f in zip(masks, kernels, want):
-        u_ref, _ = oracle.diffusion_f32(m, k, 4)
-        assert np.abs(f - u_ref).max() <= F32_TOL
+        if prec == "fp32":
+            u_ref, _ = oracle.diffusion_f32(m, k, 4)
+            assert np.abs(f - u_ref).max() <= F32_TOL
+        else:
+            u64 = oracle.diffusion_f64(m, k.astype(np.float64), 4)
+            assert np.abs(f - u64).max() <= LOWP_TOL
     plan.close()
 
 
@@ -365,7 +373,8 @@ def test_kernel_forms_agree_random(gpu, pass_kernel, prec):
             assert np.array_equal(u, want[1]) and np.array_equal(q, want[0])
 
 
-def test_batched_random_shapes_equal_single_volume(gpu):
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+def test_batched_random_shapes_equal_single_volume(gpu, prec):
     """Batched plans over random shapes and radii equal the single-volume path."""
     rng = np.random.default_rng(31)
     masks, kernels = [], []
@@ -373,7 +382,8 @@ def test_batched_random_shapes_equal_single_volume(gpu):
         shape = tuple(int(v) for v in rng.integers(3, 70, 3))
         masks.append((rng.random(shape) > 0.95).astype(np.uint8))
         kernels.append(gauss_kernel(float(rng.uniform(0.5, 12.0))).astype(np.float32))
-    qs, fs = gpu.diffusion_batched(masks, kernels, 3, precision="fp32", u16=True, f32=True)
+    _say("batched random shapes", prec)
+    qs, fs = gpu.diffusion_batched(masks, kernels, 3, precision=prec, u16=True, f32=True)
     for m, k, q, f in zip(masks, kernels, qs, fs):
-        q1, f1 = gpu.diffusion(m, k, 3, precision="fp32", u16=True, f32=True)
+        q1, f1 = gpu.diffusion(m, k, 3, precision=prec, u16=True, f32=True)
         assert np.array_equal(f, f1) and np.array_equal(q, q1)
