@@ -264,7 +264,9 @@ static void log_v(int verbose, int level, const std::string &msg) {
 }
 
 // Host-resident chunked driver.  hs0/hs1: host FP32 state buffers (n floats
-// each, may be pinned, pageable or file-backed).  The device holds one chunk:
+// each, may be pinned, pageable or file-backed; unused, and may be null, when
+// reps <= T).  The first block starts from u0 = (mask > 0) built on the
+// device; the last writes its state only to out_f32.  The device holds one chunk:
 // layers [lo, hi) = [z0 - H, z0 + cz + H) clipped to the volume, H = T*R.
 // Clipping keeps the true zero boundary at the volume ends; at interior chunk
 // edges the fake boundary corrupts at most R layers per repetition, so after
@@ -290,20 +292,21 @@ static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
     MXB_HIP(d0.alloc(cap * 4));
     MXB_HIP(d1.alloc(cap * 4));
     if (out_u16) MXB_HIP(dq.alloc(cap * 2));
-    // u0 = (mask > 0) on the host state
-    for (int64_t i = 0; i < n; ++i) hs0[i] = mask[i] > 0 ? 1.0f : 0.0f;
+    // The host state is first written at the end of the first block: that
+    // block builds u0 = (mask > 0) on the device from the mask chunk.
     float *cur = hs0, *nxt = hs1;
     int64_t done = 0;
     if (reps == 0) {
-        // quantise u0 chunk by chunk on the device
+        // u0 chunk by chunk on the device
         for (int64_t z0 = 0; z0 < nz; z0 += cz) {
             const int64_t c = std::min(cz, nz - z0);
-            MXB_HIP(hipMemcpyAsync(d0.p, cur + z0 * layer, c * layer * 4, hipMemcpyHostToDevice, st.s));
+            MXB_HIP(hipMemcpyAsync(dm.p, mask + z0 * layer, c * layer, hipMemcpyHostToDevice, st.s));
+            MXB_HIP(launch_init_state((const uint8_t *)dm.p, (float *)d0.p, c * layer, st.s));
             MXB_HIP(launch_quantise((float *)d0.p, out_u16 ? (uint16_t *)dq.p : nullptr, nullptr, c * layer, st.s));
             if (out_u16) MXB_HIP(hipMemcpyAsync(out_u16 + z0 * layer, dq.p, c * layer * 2, hipMemcpyDeviceToHost, st.s));
+            if (out_f32) MXB_HIP(hipMemcpyAsync(out_f32 + z0 * layer, d0.p, c * layer * 4, hipMemcpyDeviceToHost, st.s));
             MXB_HIP(hipStreamSynchronize(st.s));
         }
-        if (out_f32) std::memcpy(out_f32, cur, n * 4);
         return OK;
     }
     while (done < reps) {
@@ -316,7 +319,10 @@ static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
             const int64_t lz = hi - lo;
             log_v(verbose, 2, "chunk z0=" + std::to_string(z0) + " layers=" + std::to_string(lz));
             MXB_HIP(hipMemcpyAsync(dm.p, mask + lo * layer, lz * layer, hipMemcpyHostToDevice, st.s));
-            MXB_HIP(hipMemcpyAsync(d0.p, cur + lo * layer, lz * layer * 4, hipMemcpyHostToDevice, st.s));
+            if (done == 0)
+                MXB_HIP(launch_init_state((const uint8_t *)dm.p, (float *)d0.p, lz * layer, st.s));
+            else
+                MXB_HIP(hipMemcpyAsync(d0.p, cur + lo * layer, lz * layer * 4, hipMemcpyHostToDevice, st.s));
             MaskBits bits;
             MXB_HIP(pack_bits((const uint8_t *)dm.p, lz, ny, nx, (uint32_t *)db.p, bits, st.s));
             float *fin = nullptr;
@@ -325,7 +331,11 @@ static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
                                    nullptr, &fin, st.s);
             if (rc != OK) return rc;
             const int64_t off = (z0 - lo) * layer;
-            MXB_HIP(hipMemcpyAsync(nxt + z0 * layer, fin + off, c * layer * 4, hipMemcpyDeviceToHost, st.s));
+            // the interior's state: to the other host buffer, or after the last
+            // block to out_f32 (when wanted)
+            float *state_dst = last_block ? out_f32 : nxt;
+            if (state_dst != nullptr)
+                MXB_HIP(hipMemcpyAsync(state_dst + z0 * layer, fin + off, c * layer * 4, hipMemcpyDeviceToHost, st.s));
             if (last_block && out_u16)
                 MXB_HIP(hipMemcpyAsync(out_u16 + z0 * layer, (uint16_t *)dq.p + off, c * layer * 2,
                                        hipMemcpyDeviceToHost, st.s));
@@ -335,7 +345,6 @@ static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
         done += tb;
         log_v(verbose, 1, "repetition " + std::to_string(done) + "/" + std::to_string(reps));
     }
-    if (out_f32) std::memcpy(out_f32, cur, n * 4);
     return OK;
 }
 
@@ -364,7 +373,9 @@ static int chunked_host(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
                         uint16_t *out_u16, float *out_f32, int verbose) {
     const int64_t n = nz * ny * nx;
     HostBuf h0, h1;
-    if (!h0.alloc(n * 4) || !h1.alloc(n * 4)) {
+    // host state carries the volume between blocks of T repetitions; one
+    // block (reps <= T) needs none
+    if (reps > std::max<int64_t>(T, 1) && (!h0.alloc(n * 4) || !h1.alloc(n * 4))) {
         set_error("host allocation of the chunked state failed");
         return ENOMEM_;
     }

@@ -1,7 +1,9 @@
 // FP32 line pass, direct-load form (gfx950): the per-block bodies, shared by
 // the single-volume kernel (mxb_kernels_f32_lds.hip) and the batched
 // persistent kernel (mxb_batched.hip).  f32_dma_body runs one 16-position
-// tile per iteration, f32_dma_body2 two; f32_dma_pass picks per epilogue form.
+// tile per iteration (the line-contiguous z-pass), f32_dma_body2 two (the
+// position-contiguous y and x passes, the x-pass illuminating from mask
+// bits); f32_dma_pass picks per epilogue form.
 #pragma once
 #include "mxb_dma.h"
 #include "mxb_lds_body.h"
@@ -9,8 +11,7 @@
 namespace mxb {
 
 // Direct-load form: the same pass and the same LDS row layout, with each slot
-// (16 positions x 64 lines, 4 KB) and, for the illuminating pass, each
-// 64-line x 16-position mask tile moved global -> LDS by
+// (16 positions x 64 lines, 4 KB) moved global -> LDS by
 // global_load_lds_dwordx4, D slots ahead of use, with no VGPR staging.
 // Lane i of wave w writes LDS bytes 1024 w + 16 i of its slot: row
 // r = 4w + i/16 at float column 4 (i % 16), which holds lines
@@ -19,14 +20,13 @@ namespace mxb {
 // Iteration j (tile j): A. wait until group j has landed, barrier;
 // B. issue group j + D into the slots freed by tile j - 1; C. compute tile j;
 // D. stage the outputs in LDS, barrier, S stores per thread.  Group i = data
-// slot i + W - 1 (+ mask tile i); the wait count is (D-1) G + D S once j >= D
-// and (D-1) G before (G loads per group), as in the 16-bit direct-load pass.
+// slot i + W - 1; the wait count is (D-1) + D S once j >= D and D-1 before,
+// as in the 16-bit direct-load pass.
 template <int RT, int D>
 struct F32DmaShape {
     static constexpr int C = LdsShape<RT>::C;
     static constexpr int W = LdsShape<RT>::W;
     static constexpr int NSL = W + D;  // ring slots
-    static constexpr int NM = D + 2;   // mask tile slots (a pair's first tile is read at its flush)
 };
 
 #ifndef MXB_F32_DMA_DEPTH
@@ -39,16 +39,16 @@ constexpr int kF32StagePitchV = 36;      // floats per line row (position-contig
 // row groups one MFMA result writes (4 kg + r) in distinct 16-bank groups
 constexpr int kF32StagePitchZ = 68;
 
-// Shared-memory layout of one block of the direct-load pass (bytes):
-// [ring | mask ring | stage | tap table].
+// Shared-memory layout of one block of the one-tile pass (bytes):
+// [ring | stage | tap table].
 template <int RT, int D, bool MASK, bool ZSTAGE>
 struct F32DmaSmem {
+    static_assert(!MASK, "the illuminating pass runs on f32_dma_body2 (mask bits)");
     using S = F32DmaShape<RT, D>;
     static constexpr int RING = S::NSL * kF32Slot * 4;
-    static constexpr int MRING = MASK ? S::NM * 1024 : 0;
     static constexpr int STAGE = (ZSTAGE ? 16 * kF32StagePitchZ : 64 * kF32StagePitchV) * 4;
     static constexpr int TAPS = (4 * S::C + 32) * 4;
-    static constexpr int BYTES = RING + MRING + STAGE + TAPS;
+    static constexpr int BYTES = RING + STAGE + TAPS;
 };
 
 // One block's share of one pass: 64 lines from group `lb` of outer index
@@ -61,8 +61,7 @@ __device__ __forceinline__ void f32_dma_body(const float *__restrict__ in, const
                                              int64_t seg_id, const TapSrc &taps,
                                              unsigned char *__restrict__ smem) {
     using S = F32DmaShape<RT, D>;
-    constexpr int C = S::C, W = S::W, NSL = S::NSL, NM = S::NM;
-    constexpr int G = MASK ? 2 : 1;
+    constexpr int C = S::C, W = S::W, NSL = S::NSL;
     // stores per thread: every tile (line-contiguous) or every second tile
     // (position-contiguous: a pair of tiles makes 128-byte line pieces)
     constexpr int ST = ZSTAGE ? 1 : (OUTF ? 2 : 0) + (OUTU ? 1 : 0);
@@ -73,9 +72,8 @@ __device__ __forceinline__ void f32_dma_body(const float *__restrict__ in, const
     using M = F32DmaSmem<RT, D, MASK, ZSTAGE>;
     constexpr int TAPS = M::TAPS / 4;
     float *const ring = reinterpret_cast<float *>(smem);
-    unsigned char *const mring = smem + M::RING;
-    float *const stage = reinterpret_cast<float *>(smem + M::RING + M::MRING);
-    float *const tap_lds = reinterpret_cast<float *>(smem + M::RING + M::MRING + M::STAGE);
+    float *const stage = reinterpret_cast<float *>(smem + M::RING);
+    float *const tap_lds = reinterpret_cast<float *>(smem + M::RING + M::STAGE);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -108,20 +106,7 @@ __device__ __forceinline__ void f32_dma_body(const float *__restrict__ in, const
                              : static_cast<const void *>(g_dma_zero);
         dma16<ZSTAGE>(ring_lds + (uint32_t)((slot % NSL) * kF32Slot * 4), src);
     };
-    // ---- mask loads (lanes < 16): line 16 wv + lane, 16 positions
-    const bool ml_ok = MASK && line0 + 16 * wv + lane < g.n_lane;
-    const uint8_t *msrc = MASK ? e.mask + outer * g.out_outer + (line0 + 16 * wv + lane) * g.out_lane : nullptr;
-    const uint32_t mring_lds = lds_addr(mring) + 256u * (uint32_t)wv;
-    auto issue_mask = [&](int tj) {
-        const int64_t tpos = a0 + 16 * (int64_t)tj;
-        const bool ok = ml_ok && tpos + 16 <= a1;
-        const void *src = ok ? static_cast<const void *>(msrc + tpos) : static_cast<const void *>(g_dma_zero);
-        if (lane < 16) dma16<false>(mring_lds + (uint32_t)((tj % NM) * 1024), src);
-    };
-    auto issue_group = [&](int i) {
-        issue_data(i + W - 1);
-        if (MASK) issue_mask(i);
-    };
+    auto issue_group = [&](int i) { issue_data(i + W - 1); };
 
     for (int s = 0; s < W - 1; ++s) issue_data(s);
     for (int i = 0; i < D; ++i) issue_group(i);
@@ -142,9 +127,9 @@ __device__ __forceinline__ void f32_dma_body(const float *__restrict__ in, const
 
     for (int j = 0; j < ntiles; ++j) {
         if (j >= D)
-            vm_wait<(D - 1) * G + (ZSTAGE ? D : D / 2) * ST>();
+            vm_wait<(D - 1) + (ZSTAGE ? D : D / 2) * ST>();
         else
-            vm_wait<(D - 1) * G>();
+            vm_wait<D - 1>();
         __syncthreads();
         issue_group(j + D);
         int sb[W];
@@ -186,16 +171,6 @@ __device__ __forceinline__ void f32_dma_body(const float *__restrict__ in, const
                 const int64_t p0 = ppos + 8 * sq;
                 const int64_t off = outer * g.out_outer + (line0 + sl) * g.out_lane + p0;
                 if (pfull && half == 1) {
-                    if (MASK) {
-                        // positions 8 sq .. +7 of the pair: tile j-1 for sq < 2, else tile j
-                        const int tj = sq < 2 ? j - 1 : j;
-                        const uint2 m = *reinterpret_cast<const uint2 *>(mring + (tj % NM) * 1024 + sl * 16 + 8 * (sq & 1));
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            if ((m.x >> (8 * r)) & 0xffu) v[0][r] = 1.0f;
-                            if ((m.y >> (8 * r)) & 0xffu) v[1][r] = 1.0f;
-                        }
-                    }
                     if (OUTF) {
                         float *dst = sl_ok ? e.out_f32 + off : sinkf;
                         store16<false>(dst, v[0]);
@@ -215,8 +190,7 @@ __device__ __forceinline__ void f32_dma_body(const float *__restrict__ in, const
 #pragma unroll
                     for (int r = 0; r < 8; ++r) {
                         if (p0 + r >= a1 || p0 + r >= tpos + 16) break;
-                        float w = v[r >> 2][r & 3];
-                        if (MASK && gmem(e.mask)[off + r] != 0) w = 1.0f;
+                        const float w = v[r >> 2][r & 3];
                         if (OUTF) gmem(e.out_f32)[off + r] = w;
                         if (OUTU) gmem(e.out_u16)[off + r] = quantise_u16_l(w);
                     }
@@ -239,10 +213,10 @@ __device__ __forceinline__ void f32_dma_body(const float *__restrict__ in, const
 // feeds both accumulation chains (each chain still adds its taps in order:
 // the results are bit-identical to the one-tile form).
 //
-// Group i = data slots 2i + W - 1, 2i + W (+ the mask tile pair i, 64 lines x
-// 32 positions).  Pair j reads slots 2j .. 2j + W; the ring holds W + 1 + 2D
-// slots; the wait before pair j is vmcnt((D-1) G + D S) once j >= D, with
-// G = 2 (+1 mask) loads per group and S stores per pair.
+// Group i = data slots 2i + W - 1, 2i + W (+, when i % 4 == 0, the mask words
+// of pairs i .. i+3).  Pair j reads slots 2j .. 2j + W; the ring holds
+// W + 1 + 2D slots; the wait before pair j is vmcnt(GW + D S) once j >= D,
+// with GW the loads of D - 1 groups and S stores per pair.
 template <int RT, int D>
 struct F32Dma2Shape {
     static constexpr int C = LdsShape<RT>::C;
@@ -450,8 +424,13 @@ __device__ __forceinline__ void f32_dma_body2(const float *__restrict__ in, cons
 // group in flight), which amortises their staged epilogue over twice the work.
 template <int RT, int D, bool MASK, bool ZSTAGE>
 struct F32DmaForm {
-    static constexpr int BYTES = ZSTAGE ? F32DmaSmem<RT, D, MASK, ZSTAGE>::BYTES
-                                        : F32Dma2Smem<RT, 1, MASK, ZSTAGE>::BYTES;
+    static constexpr int bytes() {
+        if constexpr (ZSTAGE)
+            return F32DmaSmem<RT, D, false, true>::BYTES;  // the line-contiguous pass never illuminates
+        else
+            return F32Dma2Smem<RT, 1, MASK, false>::BYTES;
+    }
+    static constexpr int BYTES = bytes();
 };
 
 template <int RT, int D, bool MASK, bool OUTF, bool OUTU, bool ZSTAGE, typename TapSrc>

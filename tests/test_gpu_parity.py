@@ -146,6 +146,23 @@ def test_chunked_equals_in_memory(gpu, chunk_z, temporal):
     assert np.array_equal(gpu.diffusion_chunked(m, k, 5, chunk_z, temporal, "fp32_exact", f32=True)[1], u_ref)
 
 
+@pytest.mark.parametrize("reps,temporal", [(0, 1), (3, 3), (3, 5), (4, 1), (7, 3)])
+def test_chunked_single_block_and_outputs(gpu, reps, temporal):
+    """One block of repetitions (reps <= temporal: no host state), zero
+    repetitions, and each output alone, against the in-memory path."""
+    rng = np.random.default_rng(40 + reps * 7 + temporal)
+    m = (rng.random((29, 21, 40)) > 0.96).astype(np.uint8)
+    k = gauss_kernel(1.6).astype(np.float32)
+    _say("chunked outputs", reps, temporal)
+    q0, u0 = gpu.diffusion(m, k, reps, f32=True)
+    q1, u1 = gpu.diffusion_chunked(m, k, reps, 8, temporal, f32=True)
+    assert np.array_equal(u0, u1) and np.array_equal(q0, q1)
+    q2, _ = gpu.diffusion_chunked(m, k, reps, 8, temporal, u16=True, f32=False)
+    assert np.array_equal(q2, q0)
+    _, u3 = gpu.diffusion_chunked(m, k, reps, 8, temporal, u16=False, f32=True)
+    assert np.array_equal(u3, u0)
+
+
 def test_on_disk_overload(gpu, tmp_path):
     from maxibone_amd.diffusion import diffusion
     rng = np.random.default_rng(21)
