@@ -284,14 +284,37 @@ static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
     const int64_t H = T * R;
     const int64_t max_layers = std::min(nz, cz + 2 * H);
     const int64_t cap = max_layers * layer;
-    Stream st;
-    MXB_HIP(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
-    DevBuf dm, d0, d1, dq, db;
-    MXB_HIP(dm.alloc(cap));
-    MXB_HIP(db.alloc(mask_bits_bytes(max_layers, ny, nx)));
-    MXB_HIP(d0.alloc(cap * 4));
-    MXB_HIP(d1.alloc(cap * 4));
-    if (out_u16) MXB_HIP(dq.alloc(cap * 2));
+    // Two buffer sets on two streams when the device has room for both:
+    // consecutive chunks alternate, so one chunk's uploads overlap the
+    // previous chunk's compute and downloads.
+    struct ChunkSet {
+        Stream st;
+        DevBuf dm, d0, d1, dq, db;
+        hipError_t alloc(int64_t cap, int64_t bits_bytes, bool u16) {
+            hipError_t e;
+            if ((e = hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking)) != hipSuccess) return e;
+            if ((e = dm.alloc(cap)) != hipSuccess || (e = db.alloc(bits_bytes)) != hipSuccess ||
+                (e = d0.alloc(cap * 4)) != hipSuccess || (e = d1.alloc(cap * 4)) != hipSuccess)
+                return e;
+            return u16 ? dq.alloc(cap * 2) : hipSuccess;
+        }
+    };
+    ChunkSet sets[2];
+    MXB_HIP(sets[0].alloc(cap, mask_bits_bytes(max_layers, ny, nx), out_u16 != nullptr));
+    int nsets = 2;
+    if (sets[1].alloc(cap, mask_bits_bytes(max_layers, ny, nx), out_u16 != nullptr) != hipSuccess) {
+        (void)hipGetLastError();
+        nsets = 1;
+    }
+    auto sync_all = [&]() -> hipError_t {
+        for (int i = 0; i < nsets; ++i) {
+            hipError_t e = hipStreamSynchronize(sets[i].st.s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    Stream &st = sets[0].st;
+    DevBuf &dm = sets[0].dm, &d0 = sets[0].d0, &dq = sets[0].dq;
     // The host state is first written at the end of the first block: that
     // block builds u0 = (mask > 0) on the device from the mask chunk.
     float *cur = hs0, *nxt = hs1;
@@ -312,35 +335,42 @@ static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
     while (done < reps) {
         const int64_t tb = std::min(T, reps - done);
         const bool last_block = done + tb == reps;
-        for (int64_t z0 = 0; z0 < nz; z0 += cz) {
+        int ci = 0;
+        for (int64_t z0 = 0; z0 < nz; z0 += cz, ++ci) {
+            ChunkSet &S = sets[ci % nsets];  // its previous chunk's work precedes on its stream
+            const hipStream_t cs = S.st.s;
             const int64_t c = std::min(cz, nz - z0);
             const int64_t lo = std::max<int64_t>(0, z0 - tb * R);
             const int64_t hi = std::min(nz, z0 + c + tb * R);
             const int64_t lz = hi - lo;
             log_v(verbose, 2, "chunk z0=" + std::to_string(z0) + " layers=" + std::to_string(lz));
-            MXB_HIP(hipMemcpyAsync(dm.p, mask + lo * layer, lz * layer, hipMemcpyHostToDevice, st.s));
+            MXB_HIP(hipMemcpyAsync(S.dm.p, mask + lo * layer, lz * layer, hipMemcpyHostToDevice, cs));
             if (done == 0)
-                MXB_HIP(launch_init_state((const uint8_t *)dm.p, (float *)d0.p, lz * layer, st.s));
+                MXB_HIP(launch_init_state((const uint8_t *)S.dm.p, (float *)S.d0.p, lz * layer, cs));
             else
-                MXB_HIP(hipMemcpyAsync(d0.p, cur + lo * layer, lz * layer * 4, hipMemcpyHostToDevice, st.s));
+                MXB_HIP(hipMemcpyAsync(S.d0.p, cur + lo * layer, lz * layer * 4, hipMemcpyHostToDevice, cs));
             MaskBits bits;
-            MXB_HIP(pack_bits((const uint8_t *)dm.p, lz, ny, nx, (uint32_t *)db.p, bits, st.s));
+            MXB_HIP(pack_bits((const uint8_t *)S.dm.p, lz, ny, nx, (uint32_t *)S.db.p, bits, cs));
             float *fin = nullptr;
-            int rc = run_steps_f32((const uint8_t *)dm.p, bits, lz, ny, nx, k, K, tb, arith, (float *)d0.p,
-                                   (float *)d1.p, false, last_block && out_u16 ? (uint16_t *)dq.p : nullptr,
-                                   nullptr, &fin, st.s);
-            if (rc != OK) return rc;
+            int rc = run_steps_f32((const uint8_t *)S.dm.p, bits, lz, ny, nx, k, K, tb, arith, (float *)S.d0.p,
+                                   (float *)S.d1.p, false, last_block && out_u16 ? (uint16_t *)S.dq.p : nullptr,
+                                   nullptr, &fin, cs);
+            if (rc != OK) {
+                (void)sync_all();
+                return rc;
+            }
             const int64_t off = (z0 - lo) * layer;
             // the interior's state: to the other host buffer, or after the last
             // block to out_f32 (when wanted)
             float *state_dst = last_block ? out_f32 : nxt;
             if (state_dst != nullptr)
-                MXB_HIP(hipMemcpyAsync(state_dst + z0 * layer, fin + off, c * layer * 4, hipMemcpyDeviceToHost, st.s));
+                MXB_HIP(hipMemcpyAsync(state_dst + z0 * layer, fin + off, c * layer * 4, hipMemcpyDeviceToHost, cs));
             if (last_block && out_u16)
-                MXB_HIP(hipMemcpyAsync(out_u16 + z0 * layer, (uint16_t *)dq.p + off, c * layer * 2,
-                                       hipMemcpyDeviceToHost, st.s));
-            MXB_HIP(hipStreamSynchronize(st.s));
+                MXB_HIP(hipMemcpyAsync(out_u16 + z0 * layer, (uint16_t *)S.dq.p + off, c * layer * 2,
+                                       hipMemcpyDeviceToHost, cs));
         }
+        // the next block reads what this one wrote
+        MXB_HIP(sync_all());
         std::swap(cur, nxt);
         done += tb;
         log_v(verbose, 1, "repetition " + std::to_string(done) + "/" + std::to_string(reps));
