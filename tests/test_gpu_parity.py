@@ -163,6 +163,33 @@ def test_chunked_single_block_and_outputs(gpu, reps, temporal):
     assert np.array_equal(u3, u0)
 
 
+def test_drop_in_large_volume_staged_copies(gpu):
+    """Host buffers of 128 MB and more travel through the pinned staging
+    chunks (mask in, uint16 and float32 out, a ragged last chunk): the drop-in
+    call equals the device-resident entry bit for bit."""
+    from maxibone_amd import hip
+    rng = np.random.default_rng(77)
+    shape = (520, 512, 512)
+    m = np.zeros(shape, np.uint8)
+    m[rng.random(shape) > 0.9995] = 1
+    m[:, 200:260, 300:330] = 1
+    k = gauss_kernel(1.0).astype(np.float32)
+    _say("staged copies", shape)
+    q, u = gpu.diffusion(m, k, 1, f32=True)
+    L = gpu.lib()
+    sb = L.mxb_state_bytes(gpu.PRECISION["fp32"], *shape)
+    dm = hip.DeviceBuffer.from_array(m)
+    s0, s1 = hip.DeviceBuffer(sb), hip.DeviceBuffer(sb)
+    dq, df = hip.DeviceBuffer(2 * m.size), hip.DeviceBuffer(4 * m.size)
+    gpu.check(L.mxb_device_diffusion(dm.ptr, *shape, k.ctypes.data, len(k), 1, gpu.PRECISION["fp32"],
+                                     s0.ptr, s1.ptr, dq.ptr, df.ptr, None))
+    hip.synchronize()
+    assert np.array_equal(q, dq.to_array(np.uint16, shape))
+    assert np.array_equal(u, df.to_array(np.float32, shape))
+    for b in (dm, s0, s1, dq, df):
+        b.free()
+
+
 def test_on_disk_overload(gpu, tmp_path):
     from maxibone_amd.diffusion import diffusion
     rng = np.random.default_rng(21)
