@@ -463,7 +463,7 @@ bool stage_ready() {
 
 // Host -> device (h2d) or device -> host copy of `bytes`, ordered on `s`
 // after the work already queued there; returns with the copy complete.
-static hipError_t staged_copy(void *dst, const void *src, size_t bytes, bool h2d, hipStream_t s) {
+hipError_t staged_copy(void *dst, const void *src, size_t bytes, bool h2d, hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_stage_mu);
     if (bytes < 2 * kStageChunk || !stage_ready()) {
         hipError_t e = hipMemcpyAsync(dst, src, bytes, h2d ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, s);

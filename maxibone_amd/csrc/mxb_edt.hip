@@ -700,11 +700,11 @@ int mxb_edt(const uint8_t *data, int64_t nz, int64_t ny, int64_t nx, uint32_t *s
     if ((rc = d.alloc(n)) || (rc = ws.alloc(mxb_edt_workspace_bytes(nz, ny, nx)))) return rc;
     if (sq_out != nullptr && (rc = o2.alloc(n * 4))) return rc;
     if (out != nullptr && (rc = of.alloc(n * 4))) return rc;
-    MXB_EHIP(hipMemcpy(d.p, data, n, hipMemcpyHostToDevice));
+    MXB_EHIP(staged_copy(d.p, data, n, true, nullptr));
     rc = mxb_device_edt((const uint8_t *)d.p, nz, ny, nx, (uint32_t *)o2.p, (float *)of.p, ws.p, nullptr);
     if (rc != OK) return rc;
-    if (sq_out != nullptr) MXB_EHIP(hipMemcpy(sq_out, o2.p, n * 4, hipMemcpyDeviceToHost));
-    if (out != nullptr) MXB_EHIP(hipMemcpy(out, of.p, n * 4, hipMemcpyDeviceToHost));
+    if (sq_out != nullptr) MXB_EHIP(staged_copy(sq_out, o2.p, n * 4, false, nullptr));
+    if (out != nullptr) MXB_EHIP(staged_copy(out, of.p, n * 4, false, nullptr));
     return OK;
 }
 
@@ -723,12 +723,12 @@ int mxb_edt_field(const uint8_t *implant_solid, int64_t nz, int64_t ny, int64_t 
     if ((rc = d.alloc(n)) || (rc = ws.alloc(mxb_edt_workspace_bytes(nz, ny, nx))) || (rc = o.alloc(n * 2)))
         return rc;
     if (cylinder != nullptr && (rc = c.alloc(ny * nx))) return rc;
-    MXB_EHIP(hipMemcpy(d.p, implant_solid, n, hipMemcpyHostToDevice));
+    MXB_EHIP(staged_copy(d.p, implant_solid, n, true, nullptr));
     if (cylinder != nullptr) MXB_EHIP(hipMemcpy(c.p, cylinder, ny * nx, hipMemcpyHostToDevice));
     rc = mxb_device_edt_field((const uint8_t *)d.p, nz, ny, nx, (const uint8_t *)c.p, (uint16_t *)o.p, ws.p,
                               nullptr);
     if (rc != OK) return rc;
-    MXB_EHIP(hipMemcpy(out, o.p, n * 2, hipMemcpyDeviceToHost));
+    MXB_EHIP(staged_copy(out, o.p, n * 2, false, nullptr));
     return OK;
 }
 

@@ -594,14 +594,14 @@ int mxb_combine_fields(const uint16_t *gauss, const uint16_t *edt, int64_t nz, i
     if ((rc = g.alloc(n * 2)) || (rc = e.alloc(n * 2)) || (rc = o.alloc(n * 2)) || (rc = ws.alloc(256))) return rc;
     if (cylinder != nullptr && (rc = c.alloc(ny * nx))) return rc;
     hipStream_t s = nullptr;
-    MXB_FHIP(hipMemcpy(g.p, gauss, n * 2, hipMemcpyHostToDevice));
-    MXB_FHIP(hipMemcpy(e.p, edt, n * 2, hipMemcpyHostToDevice));
+    MXB_FHIP(staged_copy(g.p, gauss, n * 2, true, nullptr));
+    MXB_FHIP(staged_copy(e.p, edt, n * 2, true, nullptr));
     if (cylinder != nullptr) MXB_FHIP(hipMemcpy(c.p, cylinder, ny * nx, hipMemcpyHostToDevice));
     rc = device_combine((const uint16_t *)g.p, (const uint16_t *)e.p, nz, ny, nx,
                         cylinder != nullptr ? (const uint8_t *)c.p : nullptr, cylinder, (uint16_t *)o.p,
                         (uint32_t *)ws.p, s);
     if (rc != OK) return rc;
-    MXB_FHIP(hipMemcpy(out, o.p, n * 2, hipMemcpyDeviceToHost));
+    MXB_FHIP(staged_copy(out, o.p, n * 2, false, nullptr));
     return OK;
 }
 
@@ -640,8 +640,8 @@ int mxb_field_histogram(const uint16_t *voxels, int64_t nZ, int64_t nY, int64_t 
     if ((rc = v.alloc(n * 2)) || (rc = f.alloc(nf * 2)) || (rc = b.alloc(nb * 8)) ||
         (rc = ws.alloc(field_ws_bytes(nZ, nY, nX))))
         return rc;
-    MXB_FHIP(hipMemcpy(v.p, voxels, n * 2, hipMemcpyHostToDevice));
-    MXB_FHIP(hipMemcpy(f.p, field, nf * 2, hipMemcpyHostToDevice));
+    MXB_FHIP(staged_copy(v.p, voxels, n * 2, true, nullptr));
+    MXB_FHIP(staged_copy(f.p, field, nf * 2, true, nullptr));
     MXB_FHIP(hipMemcpy(b.p, bins, nb * 8, hipMemcpyHostToDevice));
     rc = device_field_histogram((const uint16_t *)v.p, nZ, nY, nX, (const uint16_t *)f.p, nz, ny, nx,
                                 (uint64_t *)b.p, field_bins, voxel_bins, vmin, vmax, fmin, fmax, ws.p, nullptr);
@@ -683,7 +683,7 @@ int mxb_axis_histograms(const uint16_t *voxels, int64_t nz, int64_t ny, int64_t 
         (rc = zb.alloc(Nz * voxel_bins * 8)) || (rc = rb.alloc(Nr * voxel_bins * 8)) ||
         (rc = ws.alloc(65536 * 4)))
         return rc;
-    MXB_FHIP(hipMemcpy(v.p, voxels, n * 2, hipMemcpyHostToDevice));
+    MXB_FHIP(staged_copy(v.p, voxels, n * 2, true, nullptr));
     MXB_FHIP(hipMemcpy(xb.p, x_bins, nx * voxel_bins * 8, hipMemcpyHostToDevice));
     MXB_FHIP(hipMemcpy(yb.p, y_bins, ny * voxel_bins * 8, hipMemcpyHostToDevice));
     if (Nz > 0) MXB_FHIP(hipMemcpy(zb.p, z_bins, Nz * voxel_bins * 8, hipMemcpyHostToDevice));

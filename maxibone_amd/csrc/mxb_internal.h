@@ -33,6 +33,11 @@ struct LineGeom {
     int32_t stage_out;               // 1: out_lane == 1, rows 16-byte aligned: stage tiles in LDS
 };
 
+// Host <-> device copy of `bytes`, ordered on `s` after its queued work and
+// complete on return; copies of 128 MB and more between pageable host memory
+// and the device go through pinned chunks filled or drained by host threads.
+hipError_t staged_copy(void *dst, const void *src, size_t bytes, bool h2d, hipStream_t s);
+
 // The mask packed to bits along x, words interleaved across y: bit (x mod 32)
 // of word [(z * words_per_row(nx) + x / 32) * ny + y] is (mask[z][y][x] != 0),
 // so one 32-position word of consecutive lines (y) is contiguous.  The

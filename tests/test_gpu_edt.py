@@ -116,3 +116,24 @@ def test_device_entries_match_host_entries(gpu):
     assert np.array_equal(q.to_array(np.uint16, shape), E.edt_field(implant, cyl != 0))
     for b in bufs + [sq, fl, q, ws]:
         b.free()
+
+
+def test_edt_field_large_host_copies(gpu):
+    """A volume whose host copies go through the pinned staging chunks (136 MB
+    in, 272 MB out): the host entry equals the device entry bit for bit."""
+    rng = np.random.default_rng(1100)
+    shape = (520, 512, 512)
+    implant = _rod_implant(shape, rng)
+    cyl = generate_cylinder_mask(512, 512).astype(np.uint8)
+    _say("edt field staged", shape)
+    host = edt.generate_edt_field(implant, cyl)
+    L = capi.lib()
+    dm = hip.DeviceBuffer.from_array(implant.astype(np.uint8))
+    dc = hip.DeviceBuffer.from_array(cyl)
+    dq = hip.DeviceBuffer(2 * implant.size)
+    ws = hip.DeviceBuffer(L.mxb_edt_workspace_bytes(*shape))
+    capi.check(L.mxb_device_edt_field(dm.ptr, *shape, dc.ptr, dq.ptr, ws.ptr, None))
+    hip.synchronize()
+    assert np.array_equal(host, dq.to_array(np.uint16, shape))
+    for b in (dm, dc, dq, ws):
+        b.free()
