@@ -124,7 +124,9 @@ typedef struct mxb_batch_plan mxb_batch_plan;
 /* stream: a hipStream_t; NULL makes the plan create its own stream. */
 int mxb_batch_create(const mxb_device_volume *volumes, int n, int64_t repetitions, int precision,
                      void *stream, mxb_batch_plan **plan);
-/* mode 0: the captured graph of persistent passes (FP32 precision);
+/* mode 0: the captured graph of persistent passes (FP32, and BF16 / FP16
+ * with their FP32 last repetition; bit-identical to mode 1), when every
+ * radius is at most 64 (else, and for the FMA / exact FP32 forms, mode 1);
  * mode 1: per-volume launches of the single-volume path (any precision). */
 int mxb_batch_run(mxb_batch_plan *plan, int mode);
 /* Wait for the plan's stream. */
