@@ -308,7 +308,6 @@ template <int RT, bool BF>
 static hipError_t launch_lowp_dma_rt(int form, const uint16_t *in, uint16_t *out, const uint8_t *mask,
                                      const MaskBits &bits, LineGeom g, const float *k, int R,
                                      hipStream_t s) {
-    constexpr int D = MXB_LOWP_DMA_DEPTH;
     const int64_t groups = ((g.n_lane + 63) / 64) * g.n_outer;
     const int64_t target = 256LL * 6;
     int64_t nseg = groups > 0 ? (target + groups - 1) / groups : 1;
@@ -322,12 +321,14 @@ static hipError_t launch_lowp_dma_rt(int form, const uint16_t *in, uint16_t *out
     const int64_t blocks = g.nseg * groups;
     if (blocks == 0) return hipSuccess;
     const TapsL<2 * RT + 1> t = split_taps<RT>(k, R, BF);
+    // prefetch depth per form (measured on 1024^3, R=40): the line-contiguous
+    // z-pass 5 tile pairs ahead, the position-contiguous passes 9
     if (form == 1)
-        hipLaunchKernelGGL((k_line_lowp_dma<RT, D, BF, false, true>), dim3((unsigned)blocks), dim3(256), 0, s, in, out, mask, bits, g, t);
+        hipLaunchKernelGGL((k_line_lowp_dma<RT, MXB_LOWP_DEPTH_Z, BF, false, true>), dim3((unsigned)blocks), dim3(256), 0, s, in, out, mask, bits, g, t);
     else if (form == 2)
-        hipLaunchKernelGGL((k_line_lowp_dma<RT, D, BF, false, false>), dim3((unsigned)blocks), dim3(256), 0, s, in, out, mask, bits, g, t);
+        hipLaunchKernelGGL((k_line_lowp_dma<RT, MXB_LOWP_DEPTH_YX, BF, false, false>), dim3((unsigned)blocks), dim3(256), 0, s, in, out, mask, bits, g, t);
     else
-        hipLaunchKernelGGL((k_line_lowp_dma<RT, D, BF, true, false>), dim3((unsigned)blocks), dim3(256), 0, s, in, out, mask, bits, g, t);
+        hipLaunchKernelGGL((k_line_lowp_dma<RT, MXB_LOWP_DEPTH_YX, BF, true, false>), dim3((unsigned)blocks), dim3(256), 0, s, in, out, mask, bits, g, t);
     return hipGetLastError();
 }
 

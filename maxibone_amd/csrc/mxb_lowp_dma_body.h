@@ -9,9 +9,14 @@
 #include "mxb_dma.h"
 #include "mxb_lowp.h"
 
-// Depth of the direct-load prefetch (tile pairs ahead).
-#ifndef MXB_LOWP_DMA_DEPTH
-#define MXB_LOWP_DMA_DEPTH 5
+// Depth of the direct-load prefetch (tile pairs ahead) of the single-volume
+// passes: the line-contiguous z-pass and the position-contiguous y / x passes
+// (the masked x-pass needs D - 1 to be a multiple of 4).
+#ifndef MXB_LOWP_DEPTH_Z
+#define MXB_LOWP_DEPTH_Z 5
+#endif
+#ifndef MXB_LOWP_DEPTH_YX
+#define MXB_LOWP_DEPTH_YX 9
 #endif
 
 namespace mxb {
