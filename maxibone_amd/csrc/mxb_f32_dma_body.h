@@ -32,6 +32,14 @@ struct F32DmaShape {
 #ifndef MXB_F32_DMA_DEPTH
 #define MXB_F32_DMA_DEPTH 2
 #endif
+// Pair depth of the two-tile body: the y-pass and the illuminating x-pass
+// (whose D - 1 must be a multiple of 4).
+#ifndef MXB_F32_DEPTH_Y
+#define MXB_F32_DEPTH_Y 1
+#endif
+#ifndef MXB_F32_DEPTH_X
+#define MXB_F32_DEPTH_X 1
+#endif
 
 constexpr int kF32Slot = 16 * 64;        // floats per slot
 constexpr int kF32StagePitchV = 36;      // floats per line row (position-contiguous epilogue, 2 tiles)
@@ -428,7 +436,7 @@ struct F32DmaForm {
         if constexpr (ZSTAGE)
             return F32DmaSmem<RT, D, false, true>::BYTES;  // the line-contiguous pass never illuminates
         else
-            return F32Dma2Smem<RT, 1, MASK, false>::BYTES;
+            return F32Dma2Smem<RT, MASK ? MXB_F32_DEPTH_X : MXB_F32_DEPTH_Y, MASK, false>::BYTES;
     }
     static constexpr int BYTES = bytes();
 };
@@ -440,7 +448,8 @@ __device__ __forceinline__ void f32_dma_pass(const float *__restrict__ in, const
     if constexpr (ZSTAGE)
         f32_dma_body<RT, D, MASK, OUTF, OUTU, ZSTAGE>(in, g, e, outer, lb, seg_id, taps, smem);
     else
-        f32_dma_body2<RT, 1, MASK, OUTF, OUTU, ZSTAGE>(in, g, e, outer, lb, seg_id, taps, smem);
+        f32_dma_body2<RT, MASK ? MXB_F32_DEPTH_X : MXB_F32_DEPTH_Y, MASK, OUTF, OUTU, ZSTAGE>(in, g, e, outer, lb,
+                                                                                           seg_id, taps, smem);
 }
 
 }  // namespace mxb
