@@ -50,10 +50,10 @@ static constexpr int kNumBuckets = sizeof(kBucketRT) / sizeof(kBucketRT[0]);
 // z form, and the y and x forms (the masked body needs D - 1 to be a
 // multiple of 4).
 #ifndef MXB_LOWP_BATCH_DEPTH_Z
-#define MXB_LOWP_BATCH_DEPTH_Z 1
+#define MXB_LOWP_BATCH_DEPTH_Z 5
 #endif
 #ifndef MXB_LOWP_BATCH_DEPTH
-#define MXB_LOWP_BATCH_DEPTH 1
+#define MXB_LOWP_BATCH_DEPTH 5
 #endif
 // Row pitch of the padded layout: 128 bytes of state elements (one L2 line),
 // so that the 64-line rows the passes move start on a line boundary and the
