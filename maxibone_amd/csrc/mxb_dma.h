@@ -36,6 +36,30 @@ __device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p) {
     return (__attribute__((address_space(1))) T *)p;
 }
 
+#ifndef MXB_XCD_SWIZZLE
+#define MXB_XCD_SWIZZLE 1
+#endif
+#ifndef MXB_NT_ALIGNED_ONLY
+#define MXB_NT_ALIGNED_ONLY 1
+#endif
+
+// Block b of a b_count-block grid is dispatched to XCD b % 8.  xcd_tile(b)
+// numbers the tiles so that each XCD gets one contiguous range of them:
+// neighbouring tiles, whose rows share L2 lines when a row is not 128-byte
+// aligned, then run on the same XCD at about the same time.
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t b_count) {
+    if (!MXB_XCD_SWIZZLE) return b;
+    const int64_t q = b_count / 8, rem = b_count % 8, x = b % 8;
+    return x * q + (x < rem ? x : rem) + b / 8;
+}
+
+// Whether rows of `row_bytes` from `base` may use the non-temporal store
+// policy: only whole 128-byte lines bypass L2 write combining safely.
+__device__ __forceinline__ bool nt_rows_ok(const void *base, int64_t row_bytes) {
+    if (!MXB_NT_ALIGNED_ONLY) return true;
+    return row_bytes % 128 == 0 && reinterpret_cast<uintptr_t>(base) % 128 == 0;
+}
+
 template <bool NT>
 __device__ __forceinline__ void store16(void *p, uint4 v) {
     const v4u32 x = {v.x, v.y, v.z, v.w};

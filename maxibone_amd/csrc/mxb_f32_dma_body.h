@@ -161,7 +161,10 @@ __device__ __forceinline__ void f32_dma_body(const float *__restrict__ in, const
             const int64_t p = tpos + zr;
             float *op = e.out_f32 + outer * g.out_outer + p * g.out_ax + line0 + 4 * zq;
             if (full) {
-                store16<true>(zq_ok ? op : sinkf, v);
+                if (nt_rows_ok(e.out_f32, 4 * g.out_outer))
+                    store16<true>(zq_ok ? op : sinkf, v);
+                else
+                    store16<false>(zq_ok ? op : sinkf, v);
             } else if (zq_ok && p < a1) {
                 store16<false>(op, v);
             }
@@ -370,7 +373,10 @@ __device__ __forceinline__ void f32_dma_body2(const float *__restrict__ in, cons
                 const int64_t p = ppos + row;
                 float *op = e.out_f32 + outer * g.out_outer + p * g.out_ax + line0 + 4 * zq;
                 if (full) {
-                    store16<true>(zq_ok ? op : sinkf, v);
+                    if (nt_rows_ok(e.out_f32, 4 * g.out_outer))
+                        store16<true>(zq_ok ? op : sinkf, v);
+                    else
+                        store16<false>(zq_ok ? op : sinkf, v);
                 } else if (zq_ok && p < a1) {
                     store16<false>(op, v);
                 }

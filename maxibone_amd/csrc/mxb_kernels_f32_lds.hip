@@ -22,7 +22,7 @@ template <int RT, int D, bool MASK, bool OUTF, bool OUTU, bool ZSTAGE>
 __global__ __launch_bounds__(256) void k_line_f32_dma(const float *__restrict__ in, LineGeom g,
                                                       Epilogue e, TapsF<2 * RT + 1> taps) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[F32DmaForm<RT, D, MASK, ZSTAGE>::BYTES];
-    int64_t bid = blockIdx.x;
+    int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t seg_id = bid % g.nseg;
     bid /= g.nseg;
     const int64_t nlb = (g.n_lane + 63) >> 6;

@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256) void k_line_lowp_dma(const uint16_t *__restric
                                                        MaskBits bits, LineGeom g,
                                                        TapsL<2 * RT + 1> taps) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[LowpDmaSmem<RT, D, MASK, ZSTAGE>::BYTES];
-    int64_t bid = blockIdx.x;
+    int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t seg_id = bid % g.nseg;
     bid /= g.nseg;
     const int64_t nlb = (g.n_lane + 63) >> 6;

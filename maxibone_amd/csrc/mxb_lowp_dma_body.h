@@ -220,7 +220,10 @@ __device__ __forceinline__ void lowp_dma_body(const uint16_t *__restrict__ in, u
             const int64_t p = ppos + zr;
             uint16_t *op = out + outer * g.out_outer + p * g.out_ax + line0 + 8 * zq;
             if (full) {
-                store16<true>(zq_ok ? op : sink, v);
+                if (nt_rows_ok(out, 2 * g.out_outer))
+                    store16<true>(zq_ok ? op : sink, v);
+                else
+                    store16<false>(zq_ok ? op : sink, v);
             } else if (zq_ok && p < a1) {
                 *reinterpret_cast<uint4 *>(op) = v;
             }
