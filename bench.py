@@ -41,6 +41,8 @@ def parse():
     p.add_argument("--reps", type=int, default=None, help="override the workload's repetitions")
     p.add_argument("--sigma", type=float, default=None,
                    help="override the workload's sigma (same mask; R = round(4 sigma))")
+    p.add_argument("--shape", type=int, nargs=3, default=None, metavar=("NZ", "NY", "NX"),
+                   help="override the workload's volume shape (same mask construction)")
     p.add_argument("--no-e2e", action="store_true",
                    help="skip the end-to-end timing of the drop-in call on host buffers")
     p.add_argument("--cpu-seconds", type=float, default=15.0,
@@ -457,6 +459,10 @@ def main():
         w = dataclasses.replace(w, sigma=args.sigma,
                                 description=f"{w.description}; sigma overridden to {args.sigma:g} "
                                             f"(R={round(4 * args.sigma)})")
+    if args.shape is not None:
+        w = dataclasses.replace(w, shape=tuple(args.shape),
+                                description=f"{w.description}; shape overridden to "
+                                            f"{'x'.join(str(v) for v in args.shape)}")
     reps = w.reps if args.reps is None else args.reps
     if args.chunked > 0:
         bench_chunked(args, w, reps)
@@ -519,7 +525,8 @@ def main():
         compute_peak = mfma_peak
     frac_hbm = d["GBps"] / PEAK_HBM_GBS
     frac_cmp = d["TFLOPs"] / compute_peak
-    traffic = load_traffic(args.workload, args.precision, dom)
+    # the committed counter readings are for the workload's own shape and sigma
+    traffic = None if args.shape or args.sigma else load_traffic(args.workload, args.precision, dom)
     if frac_cmp >= frac_hbm:
         roof = {"bound": "mfma", "achieved": round(d["TFLOPs"], 3), "peak": compute_peak,
                 "unit": "TFLOP/s", "frac": round(frac_cmp, 4), "traffic": traffic}
