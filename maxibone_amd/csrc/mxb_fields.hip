@@ -220,15 +220,20 @@ struct FieldHistArgs {
     const int32_t *zmap, *ymap, *xmap;   // voxel coordinate -> field coordinate
 };
 
-// Field histogram.  A block takes kFlushItems consecutive (row, 1024-voxel
-// chunk) items at a time (adjacent rows: similar field values and material
-// classes), gathers their (field bin, voxel bin) pairs in an LDS hash table of
-// kHashSlots keys (linear probing; at most kFlushItems * kRowChunk distinct
-// keys per flush, half the table, so a free slot or the key itself is always
-// found), then adds each distinct pair to the global bins with one atomic.
-constexpr int kFlushItems = 4;
-constexpr int kHashSlots = 2 * kFlushItems * kRowChunk;  // 8192
-constexpr int kHashShift = 19;                            // 32 - log2(kHashSlots)
+// Field histogram.  Each block takes one contiguous range of (row, 1024-voxel
+// chunk) items (adjacent rows: similar field values and material classes) and
+// gathers their (field bin, voxel bin) pairs in an LDS hash table of
+// kHashSlots keys (linear probing), counting the distinct keys it holds.  The
+// table reaches the global bins, one atomic per distinct pair, when it holds
+// more than kFlushAt keys after an item, and at the end of the range.  An item
+// adds at most kRowChunk keys, so the table is never more than half full and
+// a free slot or the key itself is always found.  Global atomics are the cost
+// here (many blocks add to the same bins): flushing every 4 items took 129 ms
+// on 1024^3 (4096 x 2048 bins), of which 20.7 ms without the flush.
+constexpr int kHashSlots = 8192;
+constexpr int kHashShift = 19;  // 32 - log2(kHashSlots)
+constexpr uint32_t kFlushAt = kHashSlots / 2 - kRowChunk;
+constexpr unsigned kFieldHistBlocks = 512;  // two 64 KB blocks per CU of a 256-CU device
 constexpr uint32_t kEmpty = 0xffffffffu;
 static_assert(kHashSlots == (1 << (32 - kHashShift)), "hash table size");
 
@@ -238,18 +243,20 @@ __global__ __launch_bounds__(kRowThreads) void k_field_histogram(const uint16_t 
                                                                  unsigned long long *__restrict__ bins) {
     __shared__ uint32_t keys[kHashSlots];
     __shared__ uint32_t counts[kHashSlots];
+    __shared__ uint32_t n_keys;
     for (int i = threadIdx.x; i < kHashSlots; i += kRowThreads) {
         keys[i] = kEmpty;
         counts[i] = 0;
     }
+    if (threadIdx.x == 0) n_keys = 0;
     __syncthreads();
     const int64_t items = w.rows * w.chunks;
-    for (int64_t first = (int64_t)blockIdx.x * kFlushItems; first < items;
-         first += (int64_t)gridDim.x * kFlushItems) {
-        for (int64_t item = first; item < first + kFlushItems && item < items; ++item) {
-            const int64_t row = item / w.chunks;
-            const int64_t x0 = (item % w.chunks) * kRowChunk + threadIdx.x * kPerThread;
-            if (x0 >= w.nx) continue;
+    const int64_t per = (items + gridDim.x - 1) / gridDim.x;
+    const int64_t i0 = (int64_t)blockIdx.x * per, i1 = i0 + per < items ? i0 + per : items;
+    for (int64_t item = i0; item < i1; ++item) {
+        const int64_t row = item / w.chunks;
+        const int64_t x0 = (item % w.chunks) * kRowChunk + threadIdx.x * kPerThread;
+        if (x0 < w.nx) {
             uint16_t v[8];
             load8(vox + row * w.nx, x0, w.nx, vec != 0, v);
             const int64_t Y = row % a.nY, Z = row / a.nY;
@@ -267,6 +274,7 @@ __global__ __launch_bounds__(kRowThreads) void k_field_histogram(const uint16_t 
                     const uint32_t prev = atomicCAS(&keys[slot], kEmpty, key);
                     if (prev == kEmpty || prev == key) {
                         atomicAdd(&counts[slot], 1u);
+                        if (prev == kEmpty) atomicAdd(&n_keys, 1u);
                         break;
                     }
                     slot = (slot + 1) & (kHashSlots - 1);
@@ -274,15 +282,20 @@ __global__ __launch_bounds__(kRowThreads) void k_field_histogram(const uint16_t 
             }
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < kHashSlots; i += kRowThreads) {
-            const uint32_t key = keys[i];
-            if (key != kEmpty) {
-                atomicAdd(&bins[key], (unsigned long long)counts[i]);
-                keys[i] = kEmpty;
-                counts[i] = 0;
-            }
-        }
+        const bool flush = n_keys > kFlushAt || item + 1 == i1;  // uniform: read between two barriers
         __syncthreads();
+        if (flush) {
+            for (int i = threadIdx.x; i < kHashSlots; i += kRowThreads) {
+                const uint32_t key = keys[i];
+                if (key != kEmpty) {
+                    atomicAdd(&bins[key], (unsigned long long)counts[i]);
+                    keys[i] = kEmpty;
+                    counts[i] = 0;
+                }
+            }
+            if (threadIdx.x == 0) n_keys = 0;
+            __syncthreads();
+        }
     }
 }
 
@@ -305,6 +318,7 @@ __device__ __forceinline__ int64_t isqrt64(int64_t s) {
 // atomic per nonzero bin: at the end of each row for y, when the plane
 // changes (rows of a block come in increasing order) for z.  x and r vary
 // along the row and take one global atomic per voxel.  LDS: 2 x vb uint32.
+template <bool XR>
 __global__ __launch_bounds__(kRowThreads) void k_axis_histograms(const uint16_t *__restrict__ vox,
                                                                  RowWalk w, AxisHistArgs a, int vec,
                                                                  unsigned long long *__restrict__ xb,
@@ -346,9 +360,11 @@ __global__ __launch_bounds__(kRowThreads) void k_axis_histograms(const uint16_t 
                 if (x >= w.nx) break;
                 const int32_t vi = a.vlut[v[k]];
                 if (vi < 0) continue;
-                atomicAdd(&xb[x * a.vb + vi], 1ull);
-                const int64_t r = isqrt64((x - a.cx) * (x - a.cx) + dy2);
-                if (r < a.Nr) atomicAdd(&rb[r * a.vb + vi], 1ull);
+                if (XR) {
+                    atomicAdd(&xb[x * a.vb + vi], 1ull);
+                    const int64_t r = isqrt64((x - a.cx) * (x - a.cx) + dy2);
+                    if (r < a.Nr) atomicAdd(&rb[r * a.vb + vi], 1ull);
+                }
                 atomicAdd(&hy[vi], 1u);
                 atomicAdd(&hz[vi], 1u);
             }
@@ -360,6 +376,85 @@ __global__ __launch_bounds__(kRowThreads) void k_axis_histograms(const uint16_t 
     if (zcur >= 0) {
         __syncthreads();
         flush(hz, zb + zcur * a.vb);
+    }
+}
+
+// Lanes of a wave that hold the same key (among those with `valid`) are
+// combined: returns true on one lane per distinct key, with `count` the
+// number of lanes that hold it.  After kMaxGroups keys the remaining lanes
+// return true with count 1 (bounded work when the keys are mostly distinct).
+// Call from wave-uniform control flow.
+constexpr int kMaxGroups = 16;
+__device__ __forceinline__ bool wave_group(uint32_t key, bool valid, uint32_t &count) {
+    const int lane = (int)__lane_id();
+    uint64_t pending = __ballot(valid);
+    bool leader = false;
+    count = 0;
+    for (int it = 0; it < kMaxGroups && pending != 0; ++it) {
+        const int l = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t k = (uint32_t)__shfl((int)key, l);
+        const uint64_t same = __ballot(valid && key == k) & pending;
+        if (lane == l) {
+            leader = true;
+            count = (uint32_t)__popcll(same);
+        }
+        pending &= ~same;
+    }
+    if ((pending >> lane) & 1ull) {
+        leader = true;
+        count = 1;
+    }
+    return leader;
+}
+
+// Axis histograms, x and r bins (vb <= kXrMaxBins).  The row kernel's lanes
+// hold different x, so each voxel cost one global atomic for x and one for r
+// (45-52 ms each on 1024^3 with 4096 bins).  Here a block takes 8 columns
+// x0 .. x0+7 and kXrThreads consecutive z (one per lane) and walks y: the
+// lanes of a wave then share x, y and therefore r, and differ only in the
+// voxel bin.  x counts gather in LDS (8 x vb uint32) and reach x_bins once
+// per block; r counts are combined over the wave (one atomic per distinct
+// voxel bin) and added to r_bins.  The y and z bins stay with the row kernel.
+constexpr int kXrThreads = 256;
+constexpr int kXrCols = 8;
+constexpr int64_t kXrMaxBins = 4096;  // LDS: kXrCols * vb * 4 bytes = 128 KB
+
+__global__ __launch_bounds__(kXrThreads) void k_axis_xr(const uint16_t *__restrict__ vox, AxisHistArgs a,
+                                                        int vec, unsigned long long *__restrict__ xb,
+                                                        unsigned long long *__restrict__ rb) {
+    extern __shared__ uint32_t hx[];  // [kXrCols][vb]
+    for (int64_t i = threadIdx.x; i < kXrCols * a.vb; i += kXrThreads) hx[i] = 0;
+    __syncthreads();
+    const int64_t ncol = (a.nx + kXrCols - 1) / kXrCols;
+    const int64_t x0 = (blockIdx.x % ncol) * kXrCols;
+    const int64_t z = (blockIdx.x / ncol) * kXrThreads + threadIdx.x;
+    const bool zok = z < a.nz && z + a.z_offset < a.Nz;
+    const uint16_t *base = vox + (zok ? z : 0) * a.ny * a.nx;
+    uint16_t cur[8] = {0, 0, 0, 0, 0, 0, 0, 0}, nxt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (zok && a.ny > 0) load8(base, x0, a.nx, vec != 0, cur);
+    for (int64_t y = 0; y < a.ny; ++y) {
+        if (zok && y + 1 < a.ny) load8(base + (y + 1) * a.nx, x0, a.nx, vec != 0, nxt);
+        const int64_t dy2 = (y - a.cy) * (y - a.cy);
+#pragma unroll
+        for (int k = 0; k < kXrCols; ++k) {
+            const int64_t x = x0 + k;
+            if (x >= a.nx) break;  // uniform
+            const int32_t vi = a.vlut[cur[k]];
+            uint32_t cnt;
+            if (wave_group((uint32_t)vi, zok && vi >= 0, cnt)) {
+                atomicAdd(&hx[k * a.vb + vi], cnt);
+                const int64_t r = isqrt64((x - a.cx) * (x - a.cx) + dy2);
+                if (r < a.Nr) atomicAdd(&rb[r * a.vb + vi], (unsigned long long)cnt);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < kXrCols * a.vb; i += kXrThreads) {
+        const uint32_t c = hx[i];
+        const int64_t x = x0 + i / a.vb;
+        if (c != 0 && x < a.nx) atomicAdd(&xb[x * a.vb + i % a.vb], (unsigned long long)c);
     }
 }
 
@@ -498,7 +593,7 @@ static int device_field_histogram(const uint16_t *vox, int64_t nZ, int64_t nY, i
     MXB_FHIP(hipMemcpyAsync(d, host.data(), host.size() * 4, hipMemcpyHostToDevice, s));
     FieldHistArgs a{nZ, nY, nX, nz, ny, nx, vb, fb, d, d + 65536, d + 2 * 65536, d + 2 * 65536 + nZ,
                     d + 2 * 65536 + nZ + nY};
-    const unsigned grid = (unsigned)std::min<int64_t>((w.rows * w.chunks + kFlushItems - 1) / kFlushItems, 4096);
+    const unsigned grid = (unsigned)std::min<int64_t>(w.rows * w.chunks, kFieldHistBlocks);
     hipLaunchKernelGGL(k_field_histogram, dim3(grid), dim3(kRowThreads), 0, s, vox, field, w, a,
                        vec_ok(vox, nX), (unsigned long long *)bins);
     MXB_FHIP(hipGetLastError());
@@ -528,10 +623,24 @@ static int device_axis_histograms(const uint16_t *vox, int64_t nz, int64_t ny, i
         return EINVAL_;
     }
     const unsigned grid = (unsigned)std::min<int64_t>(w.rows, 4096);
-    hipLaunchKernelGGL(k_axis_histograms, dim3(grid), dim3(kRowThreads), (unsigned)(2 * vb * 4), s, vox, w, a,
-                       vec_ok(vox, nx), (unsigned long long *)xb, (unsigned long long *)yb,
-                       (unsigned long long *)zb, (unsigned long long *)rb);
+    const bool columns = vb <= kXrMaxBins;
+    hipLaunchKernelGGL(columns ? k_axis_histograms<false> : k_axis_histograms<true>, dim3(grid), dim3(kRowThreads),
+                       (unsigned)(2 * vb * 4), s, vox, w, a, vec_ok(vox, nx), (unsigned long long *)xb,
+                       (unsigned long long *)yb, (unsigned long long *)zb, (unsigned long long *)rb);
     MXB_FHIP(hipGetLastError());
+    if (columns) {
+        const int64_t blocks = ((nx + kXrCols - 1) / kXrCols) * ((nz + kXrThreads - 1) / kXrThreads);
+        if (blocks > 0x7fffffffLL) {
+            set_error("axis_histograms: volume too large for one launch");
+            return EINVAL_;
+        }
+        const int smem = (int)(kXrCols * vb * 4);
+        MXB_FHIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_axis_xr),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, smem));
+        hipLaunchKernelGGL(k_axis_xr, dim3((unsigned)blocks), dim3(kXrThreads), (unsigned)smem, s,
+                           vox, a, vec_ok(vox, nx), (unsigned long long *)xb, (unsigned long long *)rb);
+        MXB_FHIP(hipGetLastError());
+    }
     MXB_FHIP(hipStreamSynchronize(s));
     return OK;
 }

@@ -82,16 +82,23 @@ def test_field_histogram_counts(gpu, case):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("case", range(4))
+@pytest.mark.parametrize("case", range(6))
 def test_axis_histograms_counts(gpu, case):
+    # case 4: more than 4096 bins (x and r counted per voxel by the row kernel);
+    # case 5: few distinct values over 300 slices (wave-combined counts, two
+    # z ranges of the column kernel, slices beyond Nz skipped)
     rng = np.random.default_rng(500 + case)
-    nz, ny, nx = [(12, 24, 28), (5, 40, 33), (20, 17, 17), (32, 64, 64)][case]
-    z_off, Nz = [(0, 12), (7, 40), (3, 15), (60, 80)][case]
-    vb = [128, 256, 64, 4096][case]
-    Nr = [20, 30, 5, 46][case]  # case 2: most radii beyond Nr are skipped
-    vrange = [(1.0, 65535.0), (2000.0, 50000.0), (1.0, 30000.0), (100.0, 65535.0)][case]
+    nz, ny, nx = [(12, 24, 28), (5, 40, 33), (20, 17, 17), (32, 64, 64), (10, 20, 24), (300, 6, 20)][case]
+    z_off, Nz = [(0, 12), (7, 40), (3, 15), (60, 80), (0, 10), (0, 290)][case]
+    vb = [128, 256, 64, 4096, 6000, 512][case]
+    Nr = [20, 30, 5, 46, 20, 12][case]  # case 2: most radii beyond Nr are skipped
+    vrange = [(1.0, 65535.0), (2000.0, 50000.0), (1.0, 30000.0), (100.0, 65535.0), (1.0, 65535.0),
+              (1.0, 65535.0)][case]
     center = (ny // 2, nx // 2)
-    vox = rng.integers(0, 65536, (nz, ny, nx), dtype=np.uint16)
+    if case == 5:
+        vox = (rng.integers(0, 4, (nz, ny, nx)) * 1000 + 500).astype(np.uint16)
+    else:
+        vox = rng.integers(0, 65536, (nz, ny, nx), dtype=np.uint16)
     vox[rng.random(vox.shape) < 0.1] = 0
     bins = [rng.integers(0, 3, (n, vb)).astype(np.uint64) for n in (nx, ny, Nz, Nr)]
     want = F.axis_histograms(vox, z_off, *[b.copy() for b in bins], center, vrange)
