@@ -220,26 +220,30 @@ struct FieldHistArgs {
     const int32_t *zmap, *ymap, *xmap;   // voxel coordinate -> field coordinate
 };
 
-// Field histogram.  Each block takes one contiguous range of (row, 1024-voxel
-// chunk) items (adjacent rows: similar field values and material classes) and
-// gathers their (field bin, voxel bin) pairs in an LDS hash table of
-// kHashSlots keys (linear probing), counting the distinct keys it holds.  The
-// table reaches the global bins, one atomic per distinct pair, when it holds
-// more than kFlushAt keys after an item, and at the end of the range.  An item
-// adds at most kRowChunk keys, so the table is never more than half full and
-// a free slot or the key itself is always found.  Global atomics are the cost
-// here (many blocks add to the same bins): flushing every 4 items took 129 ms
-// on 1024^3 (4096 x 2048 bins), of which 20.7 ms without the flush.
+// Field histogram.  The volume is walked in bricks of kBrickZ x kBrickY x
+// kBrickX voxels; each block takes one contiguous range of bricks.  A brick
+// covers a small part of the (smooth) field, so few field bins, and its
+// (field bin, voxel bin) pairs repeat: they gather in an LDS hash table of
+// kHashSlots keys (linear probing) that also counts the distinct keys it
+// holds.  The table reaches the global bins, one atomic per distinct pair,
+// when it holds more than kFlushAt keys after a step of kRowChunk voxels, and
+// at the end of the range; a step adds at most kRowChunk keys, so the table
+// is never more than half full and a free slot or the key itself is always
+// found.  Global atomics are the cost here: walking whole rows, which cross
+// the whole field, took 127 ms on 1024^3 (4096 x 2048 bins), of which 20.7 ms
+// without the atomics.
 constexpr int kHashSlots = 8192;
 constexpr int kHashShift = 19;  // 32 - log2(kHashSlots)
 constexpr uint32_t kFlushAt = kHashSlots / 2 - kRowChunk;
 constexpr unsigned kFieldHistBlocks = 512;  // two 64 KB blocks per CU of a 256-CU device
+constexpr int kBrickX = 64, kBrickY = 32, kBrickZ = 32;
+constexpr int kBrickRowsPerStep = kRowChunk / kBrickX;  // 16 brick rows of 64 voxels per step
 constexpr uint32_t kEmpty = 0xffffffffu;
 static_assert(kHashSlots == (1 << (32 - kHashShift)), "hash table size");
+static_assert(kBrickX % kPerThread == 0 && (kBrickY * kBrickZ) % kBrickRowsPerStep == 0, "brick shape");
 
-__global__ __launch_bounds__(kRowThreads) void k_field_histogram(const uint16_t *__restrict__ vox,
-                                                                 const uint16_t *__restrict__ field,
-                                                                 RowWalk w, FieldHistArgs a, int vec,
+__global__ __launch_bounds__(kRowThreads) void k_field_histogram(const uint16_t *__restrict__ vox, const uint16_t *__restrict__ field,
+                                                                 FieldHistArgs a, int vec,
                                                                  unsigned long long *__restrict__ bins) {
     __shared__ uint32_t keys[kHashSlots];
     __shared__ uint32_t counts[kHashSlots];
@@ -250,51 +254,58 @@ __global__ __launch_bounds__(kRowThreads) void k_field_histogram(const uint16_t 
     }
     if (threadIdx.x == 0) n_keys = 0;
     __syncthreads();
-    const int64_t items = w.rows * w.chunks;
-    const int64_t per = (items + gridDim.x - 1) / gridDim.x;
-    const int64_t i0 = (int64_t)blockIdx.x * per, i1 = i0 + per < items ? i0 + per : items;
-    for (int64_t item = i0; item < i1; ++item) {
-        const int64_t row = item / w.chunks;
-        const int64_t x0 = (item % w.chunks) * kRowChunk + threadIdx.x * kPerThread;
-        if (x0 < w.nx) {
-            uint16_t v[8];
-            load8(vox + row * w.nx, x0, w.nx, vec != 0, v);
-            const int64_t Y = row % a.nY, Z = row / a.nY;
-            const uint16_t *frow = field + ((int64_t)a.zmap[Z] * a.ny + a.ymap[Y]) * a.nx;
+    const int64_t bx_n = (a.nX + kBrickX - 1) / kBrickX, by_n = (a.nY + kBrickY - 1) / kBrickY;
+    const int64_t bricks = bx_n * by_n * ((a.nZ + kBrickZ - 1) / kBrickZ);
+    const int64_t per = (bricks + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = (int64_t)blockIdx.x * per, b1 = b0 + per < bricks ? b0 + per : bricks;
+    const int xo = (threadIdx.x % (kBrickX / kPerThread)) * kPerThread;
+    const int ro = threadIdx.x / (kBrickX / kPerThread);
+    constexpr int kSteps = kBrickY * kBrickZ / kBrickRowsPerStep;
+    for (int64_t b = b0; b < b1; ++b) {
+        const int64_t bx = b % bx_n, by = (b / bx_n) % by_n, bz = b / (bx_n * by_n);
+        const int64_t x0 = bx * kBrickX + xo;
+        for (int step = 0; step < kSteps; ++step) {
+            const int r = step * kBrickRowsPerStep + ro;
+            const int64_t Y = by * kBrickY + r % kBrickY, Z = bz * kBrickZ + r / kBrickY;
+            if (x0 < a.nX && Y < a.nY && Z < a.nZ) {
+                uint16_t v[8];
+                load8(vox + (Z * a.nY + Y) * a.nX, x0, a.nX, vec != 0, v);
+                const uint16_t *frow = field + ((int64_t)a.zmap[Z] * a.ny + a.ymap[Y]) * a.nx;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (x0 + k >= w.nx) break;
-                const int32_t vi = a.vlut[v[k]];
-                if (vi < 0) continue;
-                const int32_t fi = a.flut[frow[a.xmap[x0 + k]]];
-                if (fi < 0) continue;
-                const uint32_t key = (uint32_t)fi * (uint32_t)a.vb + (uint32_t)vi;
-                uint32_t slot = (key * 2654435761u) >> kHashShift;
-                while (true) {
-                    const uint32_t prev = atomicCAS(&keys[slot], kEmpty, key);
-                    if (prev == kEmpty || prev == key) {
-                        atomicAdd(&counts[slot], 1u);
-                        if (prev == kEmpty) atomicAdd(&n_keys, 1u);
-                        break;
+                for (int k = 0; k < 8; ++k) {
+                    if (x0 + k >= a.nX) break;
+                    const int32_t vi = a.vlut[v[k]];
+                    if (vi < 0) continue;
+                    const int32_t fi = a.flut[frow[a.xmap[x0 + k]]];
+                    if (fi < 0) continue;
+                    const uint32_t key = (uint32_t)fi * (uint32_t)a.vb + (uint32_t)vi;
+                    uint32_t slot = (key * 2654435761u) >> kHashShift;
+                    while (true) {
+                        const uint32_t prev = atomicCAS(&keys[slot], kEmpty, key);
+                        if (prev == kEmpty || prev == key) {
+                            atomicAdd(&counts[slot], 1u);
+                            if (prev == kEmpty) atomicAdd(&n_keys, 1u);
+                            break;
+                        }
+                        slot = (slot + 1) & (kHashSlots - 1);
                     }
-                    slot = (slot + 1) & (kHashSlots - 1);
                 }
             }
-        }
-        __syncthreads();
-        const bool flush = n_keys > kFlushAt || item + 1 == i1;  // uniform: read between two barriers
-        __syncthreads();
-        if (flush) {
-            for (int i = threadIdx.x; i < kHashSlots; i += kRowThreads) {
-                const uint32_t key = keys[i];
-                if (key != kEmpty) {
-                    atomicAdd(&bins[key], (unsigned long long)counts[i]);
-                    keys[i] = kEmpty;
-                    counts[i] = 0;
-                }
-            }
-            if (threadIdx.x == 0) n_keys = 0;
             __syncthreads();
+            const bool flush = n_keys > kFlushAt || (b + 1 == b1 && step + 1 == kSteps);  // uniform: between two barriers
+            __syncthreads();
+            if (flush) {
+                for (int i = threadIdx.x; i < kHashSlots; i += kRowThreads) {
+                    const uint32_t key = keys[i];
+                    if (key != kEmpty) {
+                        atomicAdd(&bins[key], (unsigned long long)counts[i]);
+                        keys[i] = kEmpty;
+                        counts[i] = 0;
+                    }
+                }
+                if (threadIdx.x == 0) n_keys = 0;
+                __syncthreads();
+            }
         }
     }
 }
@@ -593,9 +604,11 @@ static int device_field_histogram(const uint16_t *vox, int64_t nZ, int64_t nY, i
     MXB_FHIP(hipMemcpyAsync(d, host.data(), host.size() * 4, hipMemcpyHostToDevice, s));
     FieldHistArgs a{nZ, nY, nX, nz, ny, nx, vb, fb, d, d + 65536, d + 2 * 65536, d + 2 * 65536 + nZ,
                     d + 2 * 65536 + nZ + nY};
-    const unsigned grid = (unsigned)std::min<int64_t>(w.rows * w.chunks, kFieldHistBlocks);
-    hipLaunchKernelGGL(k_field_histogram, dim3(grid), dim3(kRowThreads), 0, s, vox, field, w, a,
-                       vec_ok(vox, nX), (unsigned long long *)bins);
+    const int64_t bricks = ((nX + kBrickX - 1) / kBrickX) * ((nY + kBrickY - 1) / kBrickY) *
+                           ((nZ + kBrickZ - 1) / kBrickZ);
+    const unsigned grid = (unsigned)std::min<int64_t>(bricks, kFieldHistBlocks);
+    hipLaunchKernelGGL(k_field_histogram, dim3(grid), dim3(kRowThreads), 0, s, vox, field, a, vec_ok(vox, nX),
+                       (unsigned long long *)bins);
     MXB_FHIP(hipGetLastError());
     // the host table must outlive the copy
     MXB_FHIP(hipStreamSynchronize(s));
