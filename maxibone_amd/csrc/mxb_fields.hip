@@ -590,8 +590,6 @@ static int device_field_histogram(const uint16_t *vox, int64_t nZ, int64_t nY, i
         set_error("field_histogram: invalid field shape or bin counts");
         return EINVAL_;
     }
-    const RowWalk w = row_walk(nZ * nY, nX);
-    if (int rc = check_grid(w)) return rc;
     std::vector<int32_t> host(2 * 65536 + nZ + nY + nX);
     voxel_lut(vmin, vmax, vb, host.data());
     field_lut(fmin, fmax, fb, host.data() + 65536);
