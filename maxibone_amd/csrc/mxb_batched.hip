@@ -79,13 +79,27 @@ struct BatchItem {
     int32_t vol, seg, lb, outer;
 };
 
+// Prefetch depths of the FP32 persistent kernel: the z form (tiles ahead) and
+// the y and x forms (tile pairs ahead; the masked body needs D - 1 to be a
+// multiple of 4).
+#ifndef MXB_F32_BATCH_DEPTH_Z
+#define MXB_F32_BATCH_DEPTH_Z 2
+#endif
+#ifndef MXB_F32_BATCH_DEPTH_YX
+#define MXB_F32_BATCH_DEPTH_YX 1
+#endif
+
 // Pass forms of the persistent kernel: 0 z (line-contiguous output), 1 y,
 // 2 x (position-contiguous output; x illuminates).
 template <int FORM, int RT>
 __device__ __forceinline__ void batch_body(const BatchPassDesc &d, const BatchItem &it,
                                            const float *__restrict__ taps, unsigned char *smem) {
-    f32_dma_pass<RT, MXB_F32_DMA_DEPTH, FORM == 2, true, false, FORM == 0>(
-        d.in, d.g, d.e, it.outer, it.lb, it.seg, taps + d.taps_off, smem);
+    if constexpr (FORM == 0)
+        f32_dma_body<RT, MXB_F32_BATCH_DEPTH_Z, false, true, false, true>(d.in, d.g, d.e, it.outer, it.lb, it.seg,
+                                                                          taps + d.taps_off, smem);
+    else
+        f32_dma_body2<RT, MXB_F32_BATCH_DEPTH_YX, FORM == 2, true, false, false>(d.in, d.g, d.e, it.outer, it.lb,
+                                                                                 it.seg, taps + d.taps_off, smem);
 }
 
 template <int FORM>
@@ -113,7 +127,10 @@ __global__ __launch_bounds__(256) void k_batched_pass(const BatchPassDesc *__res
 
 template <int FORM, int RT>
 static constexpr int form_smem() {
-    return F32DmaForm<RT, MXB_F32_DMA_DEPTH, FORM == 2, FORM == 0>::BYTES;
+    if constexpr (FORM == 0)
+        return F32DmaSmem<RT, MXB_F32_BATCH_DEPTH_Z, false, true>::BYTES;
+    else
+        return F32Dma2Smem<RT, MXB_F32_BATCH_DEPTH_YX, FORM == 2, false>::BYTES;
 }
 
 template <int FORM>
