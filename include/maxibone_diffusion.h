@@ -105,11 +105,12 @@ typedef struct {
 int mxb_diffusion_batched(const mxb_volume_desc *volumes, int n, int64_t repetitions,
                           int precision, int verbose);
 
-/* Device-resident batch.  A plan pads each volume into a pitched workspace,
- * builds the per-pass work lists, autotunes the persistent kernel's grid and
- * segment length on one repetition (cached per batch signature, and in the
- * file named by MXB_TUNE_FILE), and captures the whole repetition loop in a
- * HIP graph.  (No reference counterpart.) */
+/* Device-resident batch.  A plan pads each volume into a pitched workspace
+ * (rows of 128 bytes of state), builds the per-pass work lists, autotunes the
+ * persistent kernel's grid and segment length (the fastest of three timed
+ * repetitions per candidate; cached per batch signature, and in the file
+ * named by MXB_TUNE_FILE), and captures the whole repetition loop in a HIP
+ * graph.  (No reference counterpart.) */
 typedef struct {
     const uint8_t *dev_mask;  /* device, (nz, ny, nx) C-order */
     int64_t nz, ny, nx;
