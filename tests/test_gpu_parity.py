@@ -319,6 +319,21 @@ def test_precision_form_edges(gpu, prec):
             assert mx <= LOWP_TOL
 
 
+def test_unaligned_rows_direct_load(gpu):
+    """Rows of 104 voxels (416 / 208 bytes, not whole 128-byte lines): the
+    direct-load passes store them with the plain policy, in the XCD-contiguous
+    tile order, for FP32 and both 16-bit forms."""
+    rng = np.random.default_rng(104)
+    m = (rng.random((40, 40, 104)) > 0.95).astype(np.uint8)
+    k = gauss_kernel(4.0).astype(np.float32)
+    _say("unaligned rows", m.shape, "fp32")
+    _check_f32(gpu, m, k, 4, "fp32")
+    for prec in ("bf16", "fp16"):
+        _say("unaligned rows", m.shape, prec)
+        mx, _, _, _ = _lowp_errors(gpu, m, k, 6, prec)
+        assert mx <= LOWP_TOL
+
+
 @pytest.mark.parametrize("prec", ["bf16", "fp16"])
 def test_precision_form_one_repetition_is_fp32(gpu, prec):
     m = WORKLOADS["c1"].mask()[:40, :50, :60].copy()
