@@ -323,19 +323,16 @@ __device__ __forceinline__ int64_t isqrt64(int64_t s) {
     return r;
 }
 
-// Axis histograms.  A block walks whole rows (grid-stride); the voxels of a
-// row share y and z, so their bin counts gather in LDS (one histogram for the
-// row's y, one for the current z plane) and reach y_bins / z_bins as one
-// atomic per nonzero bin: at the end of each row for y, when the plane
-// changes (rows of a block come in increasing order) for z.  x and r vary
-// along the row and take one global atomic per voxel.  LDS: 2 x vb uint32.
-template <bool XR>
+// Axis histograms, y and z bins.  A block walks whole rows (grid-stride); the
+// voxels of a row share y and z, so their bin counts gather in LDS (one
+// histogram for the row's y, one for the current z plane) and reach y_bins /
+// z_bins as one atomic per nonzero bin: at the end of each row for y, when
+// the plane changes (rows of a block come in increasing order) for z.  x and
+// r vary along the row; k_axis_xr counts them.  LDS: 2 x vb uint32.
 __global__ __launch_bounds__(kRowThreads) void k_axis_histograms(const uint16_t *__restrict__ vox,
                                                                  RowWalk w, AxisHistArgs a, int vec,
-                                                                 unsigned long long *__restrict__ xb,
                                                                  unsigned long long *__restrict__ yb,
-                                                                 unsigned long long *__restrict__ zb,
-                                                                 unsigned long long *__restrict__ rb) {
+                                                                 unsigned long long *__restrict__ zb) {
     extern __shared__ uint32_t hist[];  // [0, vb): row (y); [vb, 2 vb): plane (z)
     uint32_t *hy = hist, *hz = hist + a.vb;
     for (int64_t i = threadIdx.x; i < 2 * a.vb; i += kRowThreads) hist[i] = 0;
@@ -361,7 +358,6 @@ __global__ __launch_bounds__(kRowThreads) void k_axis_histograms(const uint16_t 
             }
             zcur = z;
         }
-        const int64_t dy2 = (y - a.cy) * (y - a.cy);
         for (int64_t x0 = (int64_t)threadIdx.x * kPerThread; x0 < w.nx; x0 += kRowChunk) {
             uint16_t v[8];
             load8(vox + row * w.nx, x0, w.nx, vec != 0, v);
@@ -371,11 +367,6 @@ __global__ __launch_bounds__(kRowThreads) void k_axis_histograms(const uint16_t 
                 if (x >= w.nx) break;
                 const int32_t vi = a.vlut[v[k]];
                 if (vi < 0) continue;
-                if (XR) {
-                    atomicAdd(&xb[x * a.vb + vi], 1ull);
-                    const int64_t r = isqrt64((x - a.cx) * (x - a.cx) + dy2);
-                    if (r < a.Nr) atomicAdd(&rb[r * a.vb + vi], 1ull);
-                }
                 atomicAdd(&hy[vi], 1u);
                 atomicAdd(&hz[vi], 1u);
             }
@@ -419,22 +410,41 @@ __device__ __forceinline__ bool wave_group(uint32_t key, bool valid, uint32_t &c
     return leader;
 }
 
-// Axis histograms, x and r bins (vb <= kXrMaxBins).  The row kernel's lanes
-// hold different x, so each voxel cost one global atomic for x and one for r
-// (45-52 ms each on 1024^3 with 4096 bins).  Here a block takes 8 columns
-// x0 .. x0+7 and kXrThreads consecutive z (one per lane) and walks y: the
-// lanes of a wave then share x, y and therefore r, and differ only in the
-// voxel bin.  x counts gather in LDS (8 x vb uint32) and reach x_bins once
-// per block; r counts are combined over the wave (one atomic per distinct
-// voxel bin) and added to r_bins.  The y and z bins stay with the row kernel.
+// Axis histograms, x and r bins.  Counted along rows, the lanes would hold
+// different x, so each voxel cost one global atomic for x and one for r
+// (45-52 ms each on 1024^3 with 4096 bins).  Here a block takes COLS columns
+// x0 .. x0+COLS-1 and kXrThreads consecutive z (one per lane) and walks y:
+// the lanes of a wave then share x, y and therefore r, and differ only in the
+// voxel bin.  x counts gather in LDS (COLS x vb uint32, at most kXrLds
+// bytes: 8 columns up to 4096 bins, 4 up to 8192) and reach x_bins once per
+// block; r counts are combined over the wave (one atomic per distinct voxel
+// bin) and added to r_bins.
 constexpr int kXrThreads = 256;
-constexpr int kXrCols = 8;
-constexpr int64_t kXrMaxBins = 4096;  // LDS: kXrCols * vb * 4 bytes = 128 KB
+constexpr int64_t kXrLds = 128 * 1024;
 
+// COLS (4 or 8) consecutive uint16 of a row starting at x0 (past nx read 0).
+template <int COLS>
+__device__ __forceinline__ void load_cols(const uint16_t *__restrict__ row, int64_t x0, int64_t nx, bool vec,
+                                          uint16_t v[8]) {
+    if constexpr (COLS == 8) {
+        load8(row, x0, nx, vec, v);
+    } else if (vec && x0 + 4 <= nx) {
+        const uint2 q = *reinterpret_cast<const uint2 *>(row + x0);
+        const uint32_t w[2] = {q.x, q.y};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = (uint16_t)(w[k >> 1] >> (16 * (k & 1)));
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = x0 + k < nx ? row[x0 + k] : (uint16_t)0;
+    }
+}
+
+template <int COLS>
 __global__ __launch_bounds__(kXrThreads) void k_axis_xr(const uint16_t *__restrict__ vox, AxisHistArgs a,
                                                         int vec, unsigned long long *__restrict__ xb,
                                                         unsigned long long *__restrict__ rb) {
-    extern __shared__ uint32_t hx[];  // [kXrCols][vb]
+    constexpr int kXrCols = COLS;
+    extern __shared__ uint32_t hx[];  // [COLS][vb]
     for (int64_t i = threadIdx.x; i < kXrCols * a.vb; i += kXrThreads) hx[i] = 0;
     __syncthreads();
     const int64_t ncol = (a.nx + kXrCols - 1) / kXrCols;
@@ -443,9 +453,9 @@ __global__ __launch_bounds__(kXrThreads) void k_axis_xr(const uint16_t *__restri
     const bool zok = z < a.nz && z + a.z_offset < a.Nz;
     const uint16_t *base = vox + (zok ? z : 0) * a.ny * a.nx;
     uint16_t cur[8] = {0, 0, 0, 0, 0, 0, 0, 0}, nxt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (zok && a.ny > 0) load8(base, x0, a.nx, vec != 0, cur);
+    if (zok && a.ny > 0) load_cols<COLS>(base, x0, a.nx, vec != 0, cur);
     for (int64_t y = 0; y < a.ny; ++y) {
-        if (zok && y + 1 < a.ny) load8(base + (y + 1) * a.nx, x0, a.nx, vec != 0, nxt);
+        if (zok && y + 1 < a.ny) load_cols<COLS>(base + (y + 1) * a.nx, x0, a.nx, vec != 0, nxt);
         const int64_t dy2 = (y - a.cy) * (y - a.cy);
 #pragma unroll
         for (int k = 0; k < kXrCols; ++k) {
@@ -460,7 +470,7 @@ __global__ __launch_bounds__(kXrThreads) void k_axis_xr(const uint16_t *__restri
             }
         }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
+        for (int k = 0; k < COLS; ++k) cur[k] = nxt[k];
     }
     __syncthreads();
     for (int64_t i = threadIdx.x; i < kXrCols * a.vb; i += kXrThreads) {
@@ -635,24 +645,26 @@ static int device_axis_histograms(const uint16_t *vox, int64_t nz, int64_t ny, i
         return EINVAL_;
     }
     const unsigned grid = (unsigned)std::min<int64_t>(w.rows, 4096);
-    const bool columns = vb <= kXrMaxBins;
-    hipLaunchKernelGGL(columns ? k_axis_histograms<false> : k_axis_histograms<true>, dim3(grid), dim3(kRowThreads),
-                       (unsigned)(2 * vb * 4), s, vox, w, a, vec_ok(vox, nx), (unsigned long long *)xb,
-                       (unsigned long long *)yb, (unsigned long long *)zb, (unsigned long long *)rb);
+    hipLaunchKernelGGL(k_axis_histograms, dim3(grid), dim3(kRowThreads), (unsigned)(2 * vb * 4), s, vox, w, a,
+                       vec_ok(vox, nx), (unsigned long long *)yb, (unsigned long long *)zb);
     MXB_FHIP(hipGetLastError());
-    if (columns) {
-        const int64_t blocks = ((nx + kXrCols - 1) / kXrCols) * ((nz + kXrThreads - 1) / kXrThreads);
-        if (blocks > 0x7fffffffLL) {
-            set_error("axis_histograms: volume too large for one launch");
-            return EINVAL_;
-        }
-        const int smem = (int)(kXrCols * vb * 4);
-        MXB_FHIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_axis_xr),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, smem));
-        hipLaunchKernelGGL(k_axis_xr, dim3((unsigned)blocks), dim3(kXrThreads), (unsigned)smem, s,
-                           vox, a, vec_ok(vox, nx), (unsigned long long *)xb, (unsigned long long *)rb);
-        MXB_FHIP(hipGetLastError());
+    const int cols = 8 * vb * 4 <= kXrLds ? 8 : 4;  // vb <= 8192: 4 columns fit
+    const int64_t blocks = ((nx + cols - 1) / cols) * ((nz + kXrThreads - 1) / kXrThreads);
+    if (blocks > 0x7fffffffLL) {
+        set_error("axis_histograms: volume too large for one launch");
+        return EINVAL_;
     }
+    const int smem = (int)(cols * vb * 4);
+    const void *fn = cols == 8 ? reinterpret_cast<const void *>(&k_axis_xr<8>)
+                               : reinterpret_cast<const void *>(&k_axis_xr<4>);
+    MXB_FHIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, smem));
+    if (cols == 8)
+        hipLaunchKernelGGL(k_axis_xr<8>, dim3((unsigned)blocks), dim3(kXrThreads), (unsigned)smem, s, vox, a,
+                           vec_ok(vox, nx), (unsigned long long *)xb, (unsigned long long *)rb);
+    else
+        hipLaunchKernelGGL(k_axis_xr<4>, dim3((unsigned)blocks), dim3(kXrThreads), (unsigned)smem, s, vox, a,
+                           vec_ok(vox, nx), (unsigned long long *)xb, (unsigned long long *)rb);
+    MXB_FHIP(hipGetLastError());
     MXB_FHIP(hipStreamSynchronize(s));
     return OK;
 }

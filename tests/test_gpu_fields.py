@@ -82,18 +82,20 @@ def test_field_histogram_counts(gpu, case):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("case", range(6))
+@pytest.mark.parametrize("case", range(7))
 def test_axis_histograms_counts(gpu, case):
-    # case 4: more than 4096 bins (x and r counted per voxel by the row kernel);
+    # cases 4 and 6: more than 4096 bins (4-column blocks; case 6 with the
+    # maximum 8192 bins, rows of 30 voxels and slices beyond Nz);
     # case 5: few distinct values over 300 slices (wave-combined counts, two
     # z ranges of the column kernel, slices beyond Nz skipped)
     rng = np.random.default_rng(500 + case)
-    nz, ny, nx = [(12, 24, 28), (5, 40, 33), (20, 17, 17), (32, 64, 64), (10, 20, 24), (300, 6, 20)][case]
-    z_off, Nz = [(0, 12), (7, 40), (3, 15), (60, 80), (0, 10), (0, 290)][case]
-    vb = [128, 256, 64, 4096, 6000, 512][case]
-    Nr = [20, 30, 5, 46, 20, 12][case]  # case 2: most radii beyond Nr are skipped
+    nz, ny, nx = [(12, 24, 28), (5, 40, 33), (20, 17, 17), (32, 64, 64), (10, 20, 24), (300, 6, 20),
+                  (9, 11, 30)][case]
+    z_off, Nz = [(0, 12), (7, 40), (3, 15), (60, 80), (0, 10), (0, 290), (2, 10)][case]
+    vb = [128, 256, 64, 4096, 6000, 512, 8192][case]
+    Nr = [20, 30, 5, 46, 20, 12, 9][case]  # case 2: most radii beyond Nr are skipped
     vrange = [(1.0, 65535.0), (2000.0, 50000.0), (1.0, 30000.0), (100.0, 65535.0), (1.0, 65535.0),
-              (1.0, 65535.0)][case]
+              (1.0, 65535.0), (1.0, 65535.0)][case]
     center = (ny // 2, nx // 2)
     if case == 5:
         vox = (rng.integers(0, 4, (nz, ny, nx)) * 1000 + 500).astype(np.uint16)
