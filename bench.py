@@ -6,11 +6,15 @@ and both state buffers already resident in HBM.  Default workload: 1024^3
 mask (cylinder r=300 + 2000 spheres r~U[5,40]), sigma = 10 (R = 40, 81 taps),
 40 repetitions, FP32 state; the result is written as uint16.
 
-Prints ONE JSON line.  `value` is Gvoxel*iterations/s over the K timed steps.
-`roofline` describes the dominant kernel: its average launch duration is
-measured with HIP events recorded on the launch stream inside the timed
-region.  `cpu_baseline` times the reference's own CPU module (built from its
-sources into oracle/_ref) on a bounded crop of the same workload.
+Prints ONE JSON line.  `value` is Gvoxel*iterations/s over the K timed steps,
+which run with no per-launch instrumentation.  `roofline` describes the
+dominant kernel: its average launch duration comes from a second timed region
+of K steps in which HIP events are recorded on the launch stream around every
+pass launch.  An FP32 run also times the BF16 form on the same workload
+(`bf16`, `bf16_over_fp32`) and reports both forms' error against the FP64
+oracle on a crop of the workload (`error_vs_fp64`).  `cpu_baseline` times the
+reference's own CPU module (built from its sources into oracle/_ref) on a
+bounded crop of the same workload.
 """
 import argparse
 import dataclasses
@@ -45,6 +49,8 @@ def parse():
                    help="override the workload's volume shape (same mask construction)")
     p.add_argument("--no-e2e", action="store_true",
                    help="skip the end-to-end timing of the drop-in call on host buffers")
+    p.add_argument("--no-companion", action="store_true",
+                   help="FP32 runs: skip the BF16 companion timing and the FP64 error report")
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="target CPU-baseline duration (0 disables it)")
     p.add_argument("--chunked", type=int, default=0, metavar="Z",
@@ -451,7 +457,6 @@ def main():
     if args.workload == "edt":
         bench_edt(args)
         return
-    from maxibone_amd import capi, hip
     from maxibone_amd.synth import WORKLOADS
 
     w = WORKLOADS[args.workload]
@@ -467,12 +472,57 @@ def main():
     if args.chunked > 0:
         bench_chunked(args, w, reps)
         return
-    L = capi.lib()
     mask_h = w.mask()
+    k = w.kernel().astype(np.float32)
+    main_run = device_run(args, w, mask_h, k, reps, args.precision)
+    rec = {
+        "metric": "Gvoxel*iter/s (masked 3D diffusion)",
+        "value": main_run["value"],
+        "unit": "Gvoxel*iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_run["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision.startswith("fp32") else args.precision,
+        "data": "synthetic",
+        "config": {"workload": f"{w.name}: {w.description}", "shape": list(w.shape),
+                   "sigma": w.sigma, "radius": w.radius, "repetitions": reps,
+                   "precision": args.precision, "output": "uint16"},
+        "hbm_fraction_fused_accounting": main_run["hbm_fraction_fused_accounting"],
+        "per_kernel": main_run["per_kernel"],
+        "roofline": main_run["roofline"],
+        "stream_ms_per_step": main_run["stream_ms_per_step"],
+        "instrumented_ms_per_step": main_run["instrumented_ms_per_step"],
+    }
+    if args.precision == "fp32" and not args.no_companion:
+        # the BF16 precision form on the same workload, and its speed-up
+        lo = device_run(args, w, mask_h, k, reps, "bf16")
+        rec["bf16"] = {x: lo[x] for x in ("value", "ms_per_step", "hbm_fraction_fused_accounting",
+                                          "per_kernel", "roofline")}
+        rec["bf16_over_fp32"] = round(lo["value"] / main_run["value"], 3)
+    if not args.no_e2e:
+        rec["end_to_end"] = end_to_end(w, mask_h, k, reps, args.precision)
+    if args.cpu_seconds > 0:
+        rec["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+        if not args.no_companion:
+            rec["error_vs_fp64"] = error_report(w, mask_h, k, reps)
+    else:
+        rec["cpu_baseline"] = None
+    print(json.dumps(rec), flush=True)
+
+
+def device_run(args, w, mask_h, k, reps, precision):
+    """Time `steps` full diffusions of the workload resident in HBM (after
+    `warmup` untimed ones), then, in a separate run of the same steps, the
+    per-kernel durations from HIP events on the launch stream."""
+    from maxibone_amd import capi, hip
+    L = capi.lib()
     nz, ny, nx = mask_h.shape
     n = mask_h.size
-    k = w.kernel().astype(np.float32)
-    prec = capi.PRECISION[args.precision]
+    prec = capi.PRECISION[precision]
     sbytes = L.mxb_state_bytes(prec, nz, ny, nx)
     mask_d = hip.DeviceBuffer.from_array(mask_h)
     s0 = hip.DeviceBuffer(sbytes)
@@ -487,8 +537,7 @@ def main():
     for _ in range(args.warmup):
         step()
     hip.synchronize()
-    capi.set_kernel_timing(True)
-    hip.synchronize()
+    # the headline: no per-launch instrumentation inside the timed region
     ev0, ev1 = hip.Event(), hip.Event()
     t0 = time.perf_counter()
     ev0.record(stream)
@@ -496,17 +545,29 @@ def main():
         step()
     ev1.record(stream)
     hip.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
+    elapsed = time.perf_counter() - t0
     stream_ms = ev0.elapsed_ms(ev1)
+    # per-kernel durations: one instrumented step fills the event pool, then
+    # the same number of steps is timed launch by launch
+    capi.set_kernel_timing(True)
+    step()
+    hip.synchronize()
+    capi.set_kernel_timing(True)  # resets the totals, keeps the events
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    hip.synchronize()
+    instrumented_s = time.perf_counter() - t0
     times = {name: capi.kernel_times(c) for name, c in
              (("zpass", capi.K_ZPASS), ("ypass", capi.K_YPASS), ("xpass", capi.K_XPASS),
               ("other", capi.K_OTHER))}
     capi.set_kernel_timing(False)
+    for b in (mask_d, s0, s1, out):
+        b.free()
 
     vox_iter = float(n) * reps * args.steps
     value = vox_iter / elapsed / 1e9
-    alg = algorithmic(w, args.precision)
+    alg = algorithmic(w, precision)
     per_kernel = {}
     for name in ("zpass", "ypass", "xpass"):
         ms, cnt = times[name]
@@ -518,15 +579,12 @@ def main():
                             "GBps": b * n / avg_s / 1e9, "TFLOPs": f * n / avg_s / 1e12}
     dom = max(per_kernel, key=lambda kk: per_kernel[kk]["avg_ms"])
     d = per_kernel[dom]
-    mfma_peak = PEAK_MFMA.get(args.precision.split("_")[0], PEAK_MFMA["fp32"])
-    if args.precision in ("fp32_fma", "fp32_exact"):
-        compute_peak = PEAK_VALU_FP32
-    else:
-        compute_peak = mfma_peak
+    mfma_peak = PEAK_MFMA.get(precision.split("_")[0], PEAK_MFMA["fp32"])
+    compute_peak = PEAK_VALU_FP32 if precision in ("fp32_fma", "fp32_exact") else mfma_peak
     frac_hbm = d["GBps"] / PEAK_HBM_GBS
     frac_cmp = d["TFLOPs"] / compute_peak
     # the committed counter readings are for the workload's own shape and sigma
-    traffic = None if args.shape or args.sigma else load_traffic(args.workload, args.precision, dom)
+    traffic = None if args.shape or args.sigma else load_traffic(args.workload, precision, dom)
     if frac_cmp >= frac_hbm:
         roof = {"bound": "mfma", "achieved": round(d["TFLOPs"], 3), "peak": compute_peak,
                 "unit": "TFLOP/s", "frac": round(frac_cmp, 4), "traffic": traffic}
@@ -535,48 +593,61 @@ def main():
                 "unit": "GB/s", "frac": round(frac_hbm, 4), "traffic": traffic}
     roof["kernel"] = dom
     roof["avg_launch_ms"] = round(d["avg_ms"], 4)
-    fused_bytes = (2 if args.precision in ("bf16", "fp16") else 4) * 2 + 0.125
-    rec = {
-        "metric": "Gvoxel*iter/s (masked 3D diffusion)",
+    fused_bytes = (2 if precision in ("bf16", "fp16") else 4) * 2 + 0.125
+    return {
         "value": round(value, 3),
-        "unit": "Gvoxel*iter/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
         "ms_per_step": round(elapsed / args.steps * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32" if args.precision.startswith("fp32") else args.precision,
-        "data": "synthetic",
-        "config": {"workload": f"{w.name}: {w.description}", "shape": list(w.shape),
-                   "sigma": w.sigma, "radius": w.radius, "repetitions": reps,
-                   "precision": args.precision, "output": "uint16"},
+        "stream_ms_per_step": round(stream_ms / args.steps, 3),
+        "instrumented_ms_per_step": round(instrumented_s / args.steps * 1e3, 3),
         "hbm_fraction_fused_accounting": round(value * fused_bytes / PEAK_HBM_GBS, 4),
         "per_kernel": {kk: {x: round(v, 4) if isinstance(v, float) else v for x, v in vv.items()}
                        for kk, vv in per_kernel.items()},
         "roofline": roof,
     }
-    rec["stream_ms_per_step"] = round(stream_ms / args.steps, 3)
-    for b in (mask_d, s0, s1, out):
-        b.free()
-    if not args.no_e2e:
-        # the drop-in call on host buffers: allocation, H2D of the mask, the
-        # diffusion and D2H of the uint16 result (PCIe-inclusive, not `value`)
-        q = np.empty(mask_h.shape, np.uint16)
-        t0 = time.perf_counter()
-        capi.check(L.mxb_diffusion_ex(mask_h.ctypes.data, nz, ny, nx, k.ctypes.data, len(k), reps,
-                                      prec, q.ctypes.data, None, 0))
-        t_e2e = time.perf_counter() - t0
-        rec["end_to_end"] = {"value": round(float(n) * reps / t_e2e / 1e9, 3), "unit": "Gvoxel*iter/s",
-                             "seconds": round(t_e2e, 3),
-                             "what": "one drop-in call on host buffers incl. device allocation, "
-                                     "H2D of the mask and D2H of the uint16 output"}
-    if args.cpu_seconds > 0:
-        rec["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
-    else:
-        rec["cpu_baseline"] = None
-    print(json.dumps(rec), flush=True)
+
+
+def end_to_end(w, mask_h, k, reps, precision):
+    """One drop-in call on host buffers: allocation, H2D of the mask, the
+    diffusion and D2H of the uint16 result (PCIe-inclusive, not `value`)."""
+    from maxibone_amd import capi
+    L = capi.lib()
+    nz, ny, nx = mask_h.shape
+    q = np.empty(mask_h.shape, np.uint16)
+    t0 = time.perf_counter()
+    capi.check(L.mxb_diffusion_ex(mask_h.ctypes.data, nz, ny, nx, k.ctypes.data, len(k), reps,
+                                  capi.PRECISION[precision], q.ctypes.data, None, 0))
+    t = time.perf_counter() - t0
+    return {"value": round(float(mask_h.size) * reps / t / 1e9, 3), "unit": "Gvoxel*iter/s",
+            "seconds": round(t, 3),
+            "what": "one drop-in call on host buffers incl. device allocation, "
+                    "H2D of the mask and D2H of the uint16 output"}
+
+
+def error_report(w, mask_h, k, reps, edge=112):
+    """Max-abs and RMS error of the FP32 and BF16 forms against the FP64
+    oracle, all repetitions, on a crop of the workload mask centred on the
+    cylinder surface (the oracle at full size would take hours)."""
+    import oracle
+    from maxibone_amd import capi
+    nz, ny, nx = mask_h.shape
+    c = min(edge, nz, ny, nx)
+    oz, oy = (nz - c) // 2, (ny - c) // 2
+    ox = min(max(nx // 2 + w.cyl_r - c // 2, 0), nx - c)
+    m = np.ascontiguousarray(mask_h[oz:oz + c, oy:oy + c, ox:ox + c])
+    t0 = time.perf_counter()
+    u64 = oracle.diffusion_f64(m, k.astype(np.float64), reps)
+    u32, _ = oracle.diffusion_f32(m, k, reps)
+    t = time.perf_counter() - t0
+    rep = {"crop": f"{c}^3 at (z, y, x) = ({oz}, {oy}, {ox}), centred on the cylinder surface, "
+                   f"{reps} repetitions, {len(k)} taps", "oracle_seconds": round(t, 2)}
+    for prec in ("fp32", "bf16"):
+        u = capi.diffusion(m, k, reps, precision=prec, u16=False, f32=True)[1]
+        d = u.astype(np.float64) - u64
+        rep[prec] = {"max_abs": float(f"{np.abs(d).max():.3e}"),
+                     "rms": float(f"{np.sqrt(np.mean(d * d)):.3e}")}
+        if prec == "fp32":
+            rep[prec]["max_abs_vs_fp32_oracle"] = float(f"{np.abs(u - u32).max():.3e}")
+    return rep
 
 
 if __name__ == "__main__":

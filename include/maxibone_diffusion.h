@@ -48,7 +48,8 @@ typedef enum {
  * cpu:339-374, gpu:758-806): uint8 mask in, uint16 result out, FP32 state.
  * When the volume does not fit in device memory this streams z-chunks through
  * the device, like the reference's automatic switch to
- * diffusion_out_of_core (pyb:40-45). */
+ * diffusion_out_of_core (pyb:40-45).  The environment variable
+ * MXB_DEVICE_BUDGET (bytes) lowers the device memory the switch assumes. */
 int mxb_diffusion_u8_u16(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
                          const float *kernel, int64_t kernel_size, int64_t repetitions,
                          uint16_t *output, int verbose);
@@ -70,7 +71,9 @@ int mxb_diffusion_ex(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
  * chunk_z layers plus a halo; `temporal` repetitions run per chunk visit with
  * a halo of temporal*R layers (temporal = 1 is the reference's
  * one-step-per-chunk scheme, cpu:433-453 / gpu:869-905).  Results equal the
- * in-memory form.  Replaces NS::diffusion_out_of_core (hh:74, gpu:947-1129). */
+ * in-memory form of the same precision bit for bit: BF16 / FP16 keep 16-bit
+ * host state between chunk visits and run the last repetition in FP32.
+ * Replaces NS::diffusion_out_of_core (hh:74, gpu:947-1129). */
 int mxb_diffusion_chunked(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
                           const float *kernel, int64_t kernel_size, int64_t repetitions,
                           int64_t chunk_z, int64_t temporal, int precision,
@@ -85,6 +88,26 @@ int mxb_diffusion_on_disk(const char *input_file, const float *kernel, int64_t k
                           const char *output_file, int64_t total_z, int64_t total_y,
                           int64_t total_x, int64_t global_z, int64_t global_y, int64_t global_x,
                           int64_t repetitions, int verbose);
+
+/* Flags of mxb_diffusion_on_disk_ex. */
+enum {
+    /* Illuminate, at every chunk visit, the voxels whose incoming state is
+     * exactly 1.0f instead of the input mask: the reference's on-disk rule
+     * (store_mask, src/lib/cpp/cpu/diffusion.cc:331-337 called at :445;
+     * src/lib/cpp/gpu/diffusion.cc:745-756 called at :885).  Runs one
+     * repetition per chunk visit, as the reference does, and needs an FP32
+     * precision.  Without it the result equals the in-memory form. */
+    MXB_ON_DISK_REFERENCE_MASK = 1
+};
+
+/* mxb_diffusion_on_disk with a precision (mxb_precision; the 16-bit forms keep
+ * 16-bit state files and run their last repetition in FP32) and flags (0 or
+ * MXB_ON_DISK_REFERENCE_MASK).  Replaces the same reference interface as
+ * mxb_diffusion_on_disk (pyb:67-77; cpu:376-468; gpu:808-945). */
+int mxb_diffusion_on_disk_ex(const char *input_file, const float *kernel, int64_t kernel_size,
+                             const char *output_file, int64_t total_z, int64_t total_y,
+                             int64_t total_x, int64_t global_z, int64_t global_y, int64_t global_x,
+                             int64_t repetitions, int precision, int flags, int verbose);
 
 /* ---- batched form --------------------------------------------------------- */
 

@@ -49,6 +49,8 @@ def lib():
                                       ctypes.c_int, _vp, _vp, ctypes.c_int],
             "mxb_diffusion_on_disk": [ctypes.c_char_p, _vp, _i64, ctypes.c_char_p, _i64, _i64, _i64,
                                       _i64, _i64, _i64, _i64, ctypes.c_int],
+            "mxb_diffusion_on_disk_ex": [ctypes.c_char_p, _vp, _i64, ctypes.c_char_p, _i64, _i64, _i64,
+                                         _i64, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_int],
             "mxb_diffusion_batched": [ctypes.POINTER(VolumeDesc), ctypes.c_int, _i64, ctypes.c_int,
                                       ctypes.c_int],
             "mxb_device_diffusion": [_vp, _i64, _i64, _i64, _vp, _i64, _i64, ctypes.c_int, _vp, _vp,
@@ -156,6 +158,23 @@ def diffusion_chunked(mask, kernel, reps, chunk_z, temporal=1, precision="fp32",
                                       int(chunk_z), int(temporal), PRECISION[precision], _ptr(q),
                                       _ptr(f), verbose))
     return q, f
+
+
+ON_DISK_REFERENCE_MASK = 1
+
+
+def diffusion_on_disk(input_file, kernel, output_file, total_shape, chunk_z, reps, precision="fp32",
+                      reference_mask=False, verbose=0):
+    """File-backed diffusion (mxb_diffusion_on_disk_ex): raw uint8 (z, y, x) in,
+    raw uint16 out, z-chunks of ``chunk_z`` layers.  ``reference_mask``
+    illuminates with (u == 1.0f) of each chunk's incoming state, the
+    reference's on-disk rule."""
+    k = np.ascontiguousarray(kernel, dtype=np.float32)
+    tz, ty, tx = (int(v) for v in total_shape)
+    flags = ON_DISK_REFERENCE_MASK if reference_mask else 0
+    check(lib().mxb_diffusion_on_disk_ex(os.fsencode(input_file), k.ctypes.data, k.shape[0],
+                                         os.fsencode(output_file), tz, ty, tx, int(chunk_z), ty, tx,
+                                         int(reps), PRECISION[precision], flags, verbose))
 
 
 def diffusion_batched(masks, kernels, reps, precision="fp32", u16=True, f32=False, verbose=0):

@@ -191,34 +191,51 @@ static int run_steps_f32(const uint8_t *d_mask, const MaskBits &bits, int64_t nz
     return OK;
 }
 
-// 16-bit form: repetitions 1 .. reps-1 keep BF16/FP16 state (z: s0 -> s1,
-// y: s1 -> s0, x: s0 -> s1 + illumination, swap); the state is then widened
-// to FP32 and the last repetition runs in FP32 (MFMA), so the result carries
-// no rounding of its own repetition.
-static int run_steps_lowp(const uint8_t *d_mask, const MaskBits &bits, int64_t nz, int64_t ny,
-                          int64_t nx, const float *k, int64_t K, int64_t reps, bool bf, void *s0,
-                          void *s1, uint16_t *out_u16, float *out_f32, hipStream_t s) {
+// 16-bit repetitions: n16 repetitions keep BF16/FP16 state (z: s0 -> s1,
+// y: s1 -> s0, x: s0 -> s1 + illumination, swap), starting from
+// (mask > 0) when init_from_mask, else from the 16-bit state already in s0.
+// With final_f32 the state is then widened to FP32 and one more repetition
+// runs in FP32 (MFMA), writing out_u16 / out_f32, so the result carries no
+// rounding of its own repetition; otherwise the 16-bit state is left where
+// *final16 points.
+static int run_lowp_core(const uint8_t *d_mask, const MaskBits &bits, int64_t nz, int64_t ny,
+                         int64_t nx, const float *k, int64_t K, int64_t n16, bool final_f32, bool bf,
+                         bool init_from_mask, void *s0, void *s1, uint16_t *out_u16, float *out_f32,
+                         float **final_state, uint16_t **final16, hipStream_t s) {
     const int64_t n = nz * ny * nx;
     const int R = (int)(K / 2);
     if (n == 0) return OK;
-    if (reps <= 1)
-        return run_steps_f32(d_mask, bits, nz, ny, nx, k, K, reps, Arith::MFMA, (float *)s0,
-                             (float *)s1, true, out_u16, out_f32, nullptr, s);
     uint16_t *cur = (uint16_t *)s0, *tmp = (uint16_t *)s1;
-    MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_init_lowp(d_mask, cur, n, bf, s); }));
+    if (init_from_mask) MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_init_lowp(d_mask, cur, n, bf, s); }));
     const LineGeom gz = geom_z(nz, ny, nx), gy = geom_y(nz, ny, nx), gx = geom_x(nz, ny, nx);
-    for (int64_t t = 0; t < reps - 1; ++t) {
+    for (int64_t t = 0; t < n16; ++t) {
         MXB_HIP(timed(MXB_K_ZPASS, s, [&] { return launch_line_pass_lowp(cur, tmp, nullptr, gz, k, R, bf, s); }));
         MXB_HIP(timed(MXB_K_YPASS, s, [&] { return launch_line_pass_lowp(tmp, cur, nullptr, gy, k, R, bf, s); }));
         MXB_HIP(timed(MXB_K_XPASS, s, [&] { return launch_line_pass_lowp(cur, tmp, d_mask, gx, k, R, bf, s, bits); }));
         std::swap(cur, tmp);
+    }
+    if (!final_f32) {
+        if (final16) *final16 = cur;
+        return OK;
     }
     // widen into the other buffer, then one FP32 repetition
     float *f0 = (float *)(cur == (uint16_t *)s0 ? s1 : s0);
     float *f1 = (float *)(cur == (uint16_t *)s0 ? s0 : s1);
     MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_lowp_to_f32(cur, f0, n, bf, s); }));
     return run_steps_f32(d_mask, bits, nz, ny, nx, k, K, 1, Arith::MFMA, f0, f1, false, out_u16,
-                         out_f32, nullptr, s);
+                         out_f32, final_state, s);
+}
+
+// 16-bit form of a whole diffusion: repetitions 1 .. reps-1 in BF16/FP16,
+// the last in FP32.
+static int run_steps_lowp(const uint8_t *d_mask, const MaskBits &bits, int64_t nz, int64_t ny,
+                          int64_t nx, const float *k, int64_t K, int64_t reps, bool bf, void *s0,
+                          void *s1, uint16_t *out_u16, float *out_f32, hipStream_t s) {
+    if (reps <= 1)
+        return run_steps_f32(d_mask, bits, nz, ny, nx, k, K, reps, Arith::MFMA, (float *)s0,
+                             (float *)s1, true, out_u16, out_f32, nullptr, s);
+    return run_lowp_core(d_mask, bits, nz, ny, nx, k, K, reps - 1, true, bf, true, s0, s1, out_u16,
+                         out_f32, nullptr, nullptr, s);
 }
 
 static int run_steps(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t nx, const float *k,
@@ -264,26 +281,47 @@ static void log_v(int verbose, int level, const std::string &msg) {
     }
 }
 
-// Host-resident chunked driver.  hs0/hs1: host FP32 state buffers (n floats
-// each, may be pinned, pageable or file-backed; unused, and may be null, when
-// reps <= T).  The first block starts from u0 = (mask > 0) built on the
-// device; the last writes its state only to out_f32.  The device holds one chunk:
-// layers [lo, hi) = [z0 - H, z0 + cz + H) clipped to the volume, H = T*R.
-// Clipping keeps the true zero boundary at the volume ends; at interior chunk
-// edges the fake boundary corrupts at most R layers per repetition, so after
-// T repetitions the interior [z0, z0 + cz) is exact.
+// Host-resident chunked driver.  hs0 / hs1: host state buffers of n elements
+// each (float for the FP32 forms, 16-bit for BF16 / FP16; pinned, pageable or
+// file-backed; unused, and may be null, when reps <= T).  The first block
+// starts from u0 = (mask > 0) built on the device; the last writes only the
+// outputs.  The device holds one chunk: layers [lo, hi) = [z0 - H, z0 + cz + H)
+// clipped to the volume, H = T*R.  Clipping keeps the true zero boundary at
+// the volume ends; at interior chunk edges the fake boundary corrupts at most
+// R layers per repetition, so after T repetitions the interior [z0, z0 + cz)
+// is exact.  The 16-bit forms round lo down to a multiple of 32: a 16-bit
+// pass groups a voxel's taps by its place in a 16-position tile, and with
+// that alignment every layer keeps its place, so 16-bit chunked results equal
+// in-memory ones bit for bit (FP32 passes do not depend on the tile place).
+// Each block of a 16-bit diffusion runs its repetitions in 16-bit state; the
+// last block runs its last repetition in FP32, as the in-memory form does.
+//
+// ref_mask: illuminate with (u == 1.0f) of the state each chunk visit starts
+// from instead of the input mask: the reference's on-disk path re-derives its
+// mask that way (store_mask, src/lib/cpp/cpu/diffusion.cc:331-337, :445;
+// src/lib/cpp/gpu/diffusion.cc:745-756, :885).  Needs T = 1 and FP32 state.
 static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx, const float *k,
-                        int64_t K, int64_t reps, int64_t cz, int64_t T, int precision,
-                        float *hs0, float *hs1, uint16_t *out_u16, float *out_f32, int verbose) {
+                        int64_t K, int64_t reps, int64_t cz, int64_t T, int precision, void *hs0,
+                        void *hs1, uint16_t *out_u16, float *out_f32, int verbose, bool ref_mask) {
     const int64_t R = K / 2, layer = ny * nx, n = nz * layer;
     if (n == 0) return OK;
-    if (is_lowp(precision)) precision = MXB_FP32;  // chunks carry FP32 state between visits
-    const Arith arith = arith_of(precision);
+    const bool lowp = is_lowp(precision), bf = precision == MXB_BF16;
+    if (lowp && R > kMaxRadiusLowp) {
+        set_error("BF16/FP16 forms support kernel radius <= " + std::to_string(kMaxRadiusLowp));
+        return EINVAL_;
+    }
+    if (ref_mask && (lowp || T != 1)) {
+        set_error("the reference on-disk mask rule needs FP32 state and one repetition per chunk visit");
+        return EINVAL_;
+    }
+    const Arith arith = arith_of(lowp ? MXB_FP32 : precision);
+    const int64_t esz = lowp ? 2 : 4;  // bytes per host state element
+    const int64_t align = lowp ? 32 : 1;
     if (T < 1) T = 1;
     if (cz < 1) cz = 1;
     if (cz > nz) cz = nz;
     const int64_t H = T * R;
-    const int64_t max_layers = std::min(nz, cz + 2 * H);
+    const int64_t max_layers = std::min(nz, cz + 2 * H + (align - 1));
     const int64_t cap = max_layers * layer;
     // Two buffer sets on two streams when the device has room for both:
     // consecutive chunks alternate, so one chunk's uploads overlap the
@@ -318,7 +356,7 @@ static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
     DevBuf &dm = sets[0].dm, &d0 = sets[0].d0, &dq = sets[0].dq;
     // The host state is first written at the end of the first block: that
     // block builds u0 = (mask > 0) on the device from the mask chunk.
-    float *cur = hs0, *nxt = hs1;
+    char *cur = static_cast<char *>(hs0), *nxt = static_cast<char *>(hs1);
     int64_t done = 0;
     if (reps == 0) {
         // u0 chunk by chunk on the device
@@ -341,21 +379,32 @@ static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
             ChunkSet &S = sets[ci % nsets];  // its previous chunk's work precedes on its stream
             const hipStream_t cs = S.st.s;
             const int64_t c = std::min(cz, nz - z0);
-            const int64_t lo = std::max<int64_t>(0, z0 - tb * R);
+            const int64_t lo = std::max<int64_t>(0, z0 - tb * R) / align * align;
             const int64_t hi = std::min(nz, z0 + c + tb * R);
             const int64_t lz = hi - lo;
             log_v(verbose, 2, "chunk z0=" + std::to_string(z0) + " layers=" + std::to_string(lz));
-            MXB_HIP(hipMemcpyAsync(S.dm.p, mask + lo * layer, lz * layer, hipMemcpyHostToDevice, cs));
-            if (done == 0)
-                MXB_HIP(launch_init_state((const uint8_t *)S.dm.p, (float *)S.d0.p, lz * layer, cs));
-            else
-                MXB_HIP(hipMemcpyAsync(S.d0.p, cur + lo * layer, lz * layer * 4, hipMemcpyHostToDevice, cs));
+            const bool derive_mask = ref_mask && done > 0;
+            if (!derive_mask)
+                MXB_HIP(hipMemcpyAsync(S.dm.p, mask + lo * layer, lz * layer, hipMemcpyHostToDevice, cs));
+            if (done > 0)
+                MXB_HIP(hipMemcpyAsync(S.d0.p, cur + lo * layer * esz, lz * layer * esz, hipMemcpyHostToDevice, cs));
+            if (derive_mask)
+                MXB_HIP(launch_mask_from_state((const float *)S.d0.p, (uint8_t *)S.dm.p, lz * layer, cs));
             MaskBits bits;
             MXB_HIP(pack_bits((const uint8_t *)S.dm.p, lz, ny, nx, (uint32_t *)S.db.p, bits, cs));
+            uint16_t *q = last_block && out_u16 ? (uint16_t *)S.dq.p : nullptr;
             float *fin = nullptr;
-            int rc = run_steps_f32((const uint8_t *)S.dm.p, bits, lz, ny, nx, k, K, tb, arith, (float *)S.d0.p,
-                                   (float *)S.d1.p, false, last_block && out_u16 ? (uint16_t *)S.dq.p : nullptr,
-                                   nullptr, &fin, cs);
+            uint16_t *fin16 = nullptr;
+            int rc;
+            if (lowp) {
+                rc = run_lowp_core((const uint8_t *)S.dm.p, bits, lz, ny, nx, k, K, last_block ? tb - 1 : tb,
+                                   last_block, bf, done == 0, S.d0.p, S.d1.p, q, nullptr, &fin, &fin16, cs);
+            } else {
+                if (done == 0)
+                    MXB_HIP(launch_init_state((const uint8_t *)S.dm.p, (float *)S.d0.p, lz * layer, cs));
+                rc = run_steps_f32((const uint8_t *)S.dm.p, bits, lz, ny, nx, k, K, tb, arith, (float *)S.d0.p,
+                                   (float *)S.d1.p, false, q, nullptr, &fin, cs);
+            }
             if (rc != OK) {
                 (void)sync_all();
                 return rc;
@@ -363,12 +412,17 @@ static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
             const int64_t off = (z0 - lo) * layer;
             // the interior's state: to the other host buffer, or after the last
             // block to out_f32 (when wanted)
-            float *state_dst = last_block ? out_f32 : nxt;
-            if (state_dst != nullptr)
-                MXB_HIP(hipMemcpyAsync(state_dst + z0 * layer, fin + off, c * layer * 4, hipMemcpyDeviceToHost, cs));
-            if (last_block && out_u16)
-                MXB_HIP(hipMemcpyAsync(out_u16 + z0 * layer, (uint16_t *)S.dq.p + off, c * layer * 2,
-                                       hipMemcpyDeviceToHost, cs));
+            if (last_block) {
+                if (out_f32 != nullptr)
+                    MXB_HIP(hipMemcpyAsync(out_f32 + z0 * layer, fin + off, c * layer * 4, hipMemcpyDeviceToHost, cs));
+                if (out_u16)
+                    MXB_HIP(hipMemcpyAsync(out_u16 + z0 * layer, (uint16_t *)S.dq.p + off, c * layer * 2,
+                                           hipMemcpyDeviceToHost, cs));
+            } else {
+                const char *src = lowp ? reinterpret_cast<const char *>(fin16 + off)
+                                       : reinterpret_cast<const char *>(fin + off);
+                MXB_HIP(hipMemcpyAsync(nxt + z0 * layer * esz, src, c * layer * esz, hipMemcpyDeviceToHost, cs));
+            }
         }
         // the next block reads what this one wrote
         MXB_HIP(sync_all());
@@ -403,15 +457,16 @@ static int chunked_host(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
                         int64_t K, int64_t reps, int64_t cz, int64_t T, int precision,
                         uint16_t *out_u16, float *out_f32, int verbose) {
     const int64_t n = nz * ny * nx;
+    const int64_t esz = is_lowp(precision) ? 2 : 4;
     HostBuf h0, h1;
     // host state carries the volume between blocks of T repetitions; one
     // block (reps <= T) needs none
-    if (reps > std::max<int64_t>(T, 1) && (!h0.alloc(n * 4) || !h1.alloc(n * 4))) {
+    if (reps > std::max<int64_t>(T, 1) && (!h0.alloc(n * esz) || !h1.alloc(n * esz))) {
         set_error("host allocation of the chunked state failed");
         return ENOMEM_;
     }
-    return chunked_core(mask, nz, ny, nx, k, K, reps, cz, T, precision, (float *)h0.p,
-                        (float *)h1.p, out_u16, out_f32, verbose);
+    return chunked_core(mask, nz, ny, nx, k, K, reps, cz, T, precision, h0.p, h1.p, out_u16, out_f32,
+                        verbose, false);
 }
 
 // ---------------------------------------------------------------- staged copies
@@ -501,6 +556,21 @@ hipError_t staged_copy(void *dst, const void *src, size_t bytes, bool h2d, hipSt
     return hipSuccess;
 }
 
+// Device memory the drivers may plan with: 92 % of what is free, less
+// 256 MB.  MXB_DEVICE_BUDGET (bytes) lowers it, so that small volumes can be
+// sent down the streaming paths that volumes larger than HBM take.
+static int device_budget(int64_t &budget) {
+    size_t free_b = 0, total_b = 0;
+    MXB_HIP(hipMemGetInfo(&free_b, &total_b));
+    budget = (int64_t)(free_b * 0.92) - (int64_t(256) << 20);
+    if (const char *v = std::getenv("MXB_DEVICE_BUDGET")) {
+        char *end = nullptr;
+        const long long cap = std::strtoll(v, &end, 10);
+        if (end != v && cap >= 0) budget = std::min<int64_t>(budget, cap);
+    }
+    return OK;
+}
+
 // Largest chunk depth whose device footprint fits in `budget` bytes.
 static int64_t chunk_depth_for(int64_t budget, int64_t nz, int64_t layer, int64_t H) {
     const int64_t per_layer = layer * (1 + 4 + 4 + 2);
@@ -525,11 +595,11 @@ static int in_memory(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx, co
         set_error("mask is null");
         return EINVAL_;
     }
-    size_t free_b = 0, total_b = 0;
-    MXB_HIP(hipMemGetInfo(&free_b, &total_b));
+    int64_t budget = 0;
+    rc = device_budget(budget);
+    if (rc != OK) return rc;
     const int64_t sb = state_bytes(precision, nz, ny, nx);
     const int64_t need = n + 2 * sb + (out_u16 ? 2 * n : 0) + (out_f32 ? 4 * n : 0);
-    const int64_t budget = (int64_t)(free_b * 0.92) - (int64_t(256) << 20);
     if (need > budget) {
         const int64_t H = K / 2;
         const int64_t cz = chunk_depth_for(budget, nz, ny * nx, H);
@@ -690,11 +760,11 @@ int mxb_diffusion_chunked(const uint8_t *mask, int64_t nz, int64_t ny, int64_t n
                         precision, output_u16, output_f32, verbose);
 }
 
-int mxb_diffusion_on_disk(const char *input_file, const float *kernel, int64_t kernel_size,
-                          const char *output_file, int64_t total_z, int64_t total_y,
-                          int64_t total_x, int64_t global_z, int64_t global_y, int64_t global_x,
-                          int64_t repetitions, int verbose) {
-    int rc = check_common(total_z, total_y, total_x, kernel, kernel_size, repetitions, MXB_FP32);
+int mxb_diffusion_on_disk_ex(const char *input_file, const float *kernel, int64_t kernel_size,
+                             const char *output_file, int64_t total_z, int64_t total_y,
+                             int64_t total_x, int64_t global_z, int64_t global_y, int64_t global_x,
+                             int64_t repetitions, int precision, int flags, int verbose) {
+    int rc = check_common(total_z, total_y, total_x, kernel, kernel_size, repetitions, precision);
     if (rc != OK) return rc;
     if (input_file == nullptr || output_file == nullptr) {
         set_error("file names must not be null");
@@ -704,7 +774,17 @@ int mxb_diffusion_on_disk(const char *input_file, const float *kernel, int64_t k
         set_error("global_shape must be (chunk_z >= 1, total_y, total_x): chunks split z only");
         return EINVAL_;
     }
+    if ((flags & ~MXB_ON_DISK_REFERENCE_MASK) != 0) {
+        set_error("unknown on-disk flags " + std::to_string(flags));
+        return EINVAL_;
+    }
+    const bool ref_mask = (flags & MXB_ON_DISK_REFERENCE_MASK) != 0;
+    if (ref_mask && is_lowp(precision)) {
+        set_error("the reference on-disk mask rule (u == 1.0f) needs an FP32 precision");
+        return EINVAL_;
+    }
     const int64_t n = total_z * total_y * total_x;
+    const int64_t esz = is_lowp(precision) ? 2 : 4;
     MappedFile in, out;
     rc = map_read(input_file, (size_t)n, in);
     if (rc != OK) return rc;
@@ -714,35 +794,46 @@ int mxb_diffusion_on_disk(const char *input_file, const float *kernel, int64_t k
     // State lives in two unlinked temporary files, paged by the OS (unique
     // names, so concurrent calls do not collide).
     MappedFile t0, t1;
-    rc = map_write(nullptr, (size_t)n * 4, t0, true);
+    rc = map_write(nullptr, (size_t)(n * esz), t0, true);
     if (rc != OK) return rc;
-    rc = map_write(nullptr, (size_t)n * 4, t1, true);
+    rc = map_write(nullptr, (size_t)(n * esz), t1, true);
     if (rc != OK) return rc;
-    size_t free_b = 0, total_b = 0;
-    MXB_HIP(hipMemGetInfo(&free_b, &total_b));
-    const int64_t budget = (int64_t)(free_b * 0.92) - (int64_t(256) << 20);
+    int64_t budget = 0;
+    rc = device_budget(budget);
+    if (rc != OK) return rc;
     const int64_t R = kernel_size / 2, layer = total_y * total_x;
     // The caller's chunk depth is honoured.  Temporal blocking runs T
     // repetitions per chunk visit (halo T*R layers per side), which divides
     // the file traffic by T; T is capped so that the halo stays within about
-    // one chunk depth (at most ~3x redundant work) and the chunk fits.
+    // one chunk depth (at most ~3x redundant work) and the chunk fits.  The
+    // reference mask rule re-derives the mask per visit, so it keeps T = 1.
+    const int64_t pad = is_lowp(precision) ? 31 : 0;  // the 16-bit forms align chunk starts to 32 layers
     int64_t T = 1;
-    while (T < repetitions && 2 * (T + 1) * R <= std::max(global_z, R) &&
-           std::min(total_z, global_z + 2 * (T + 1) * R) * layer * 11 < budget)
+    while (!ref_mask && T < repetitions && 2 * (T + 1) * R <= std::max(global_z, R) &&
+           std::min(total_z, global_z + 2 * (T + 1) * R + pad) * layer * 11 < budget)
         ++T;
-    if (std::min(total_z, global_z + 2 * T * R) * layer * 11 > budget) {
+    if (std::min(total_z, global_z + 2 * T * R + pad) * layer * 11 > budget) {
         set_error("chunk of " + std::to_string(global_z) + " layers plus halo exceeds device memory");
         return ENOMEM_;
     }
     rc = chunked_core((const uint8_t *)in.p, total_z, total_y, total_x, kernel, kernel_size,
-                      repetitions, global_z, T, MXB_FP32, (float *)t0.p, (float *)t1.p,
-                      (uint16_t *)out.p, nullptr, verbose);
+                      repetitions, global_z, T, precision, t0.p, t1.p, (uint16_t *)out.p, nullptr,
+                      verbose, ref_mask);
     if (rc != OK) return rc;
     if (msync(out.p, out.len, MS_SYNC) != 0) {
         set_error(std::string("Could not write the output file: ") + output_file);
         return EIO_;
     }
     return OK;
+}
+
+int mxb_diffusion_on_disk(const char *input_file, const float *kernel, int64_t kernel_size,
+                          const char *output_file, int64_t total_z, int64_t total_y,
+                          int64_t total_x, int64_t global_z, int64_t global_y, int64_t global_x,
+                          int64_t repetitions, int verbose) {
+    return mxb_diffusion_on_disk_ex(input_file, kernel, kernel_size, output_file, total_z, total_y,
+                                    total_x, global_z, global_y, global_x, repetitions, MXB_FP32, 0,
+                                    verbose);
 }
 
 int mxb_device_diffusion(const uint8_t *dev_mask, int64_t nz, int64_t ny, int64_t nx,

@@ -520,6 +520,19 @@ static unsigned reduce_grid(const RowWalk &w) {
         }                                                                       \
     } while (0)
 
+// As MXB_FHIP, for calls made after a copy from a host table was queued on
+// `s`: an error path first waits for the stream, so the copy never reads the
+// table after it is freed.
+#define MXB_FHIP_AFTER_COPY(expr, s)                                            \
+    do {                                                                        \
+        hipError_t _e = (expr);                                                 \
+        if (_e != hipSuccess) {                                                 \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(_e));       \
+            (void)hipStreamSynchronize(s);                                      \
+            return _e == hipErrorOutOfMemory ? ENOMEM_ : EDEVICE;               \
+        }                                                                       \
+    } while (0)
+
 static int check_grid(const RowWalk &w) {
     if (w.rows * w.chunks > 0x7fffffffLL) {
         set_error("volume too large for one launch of the row kernels");
@@ -618,7 +631,7 @@ static int device_field_histogram(const uint16_t *vox, int64_t nZ, int64_t nY, i
     const unsigned grid = (unsigned)std::min<int64_t>(bricks, kFieldHistBlocks);
     hipLaunchKernelGGL(k_field_histogram, dim3(grid), dim3(kRowThreads), 0, s, vox, field, a, vec_ok(vox, nX),
                        (unsigned long long *)bins);
-    MXB_FHIP(hipGetLastError());
+    MXB_FHIP_AFTER_COPY(hipGetLastError(), s);
     // the host table must outlive the copy
     MXB_FHIP(hipStreamSynchronize(s));
     return OK;
@@ -633,21 +646,13 @@ static int device_axis_histograms(const uint16_t *vox, int64_t nz, int64_t ny, i
         set_error("axis_histograms: invalid arguments");
         return EINVAL_;
     }
+    // every argument check precedes the copy of the lookup table
     const RowWalk w = row_walk(nz * ny, nx);
     if (int rc = check_grid(w)) return rc;
-    std::vector<int32_t> host(65536);
-    voxel_lut(vmin, vmax, vb, host.data());
-    int32_t *d = (int32_t *)ws;
-    MXB_FHIP(hipMemcpyAsync(d, host.data(), host.size() * 4, hipMemcpyHostToDevice, s));
-    AxisHistArgs a{nz, ny, nx, z_offset, Nz, Nr, vb, d, (int64_t)cy, (int64_t)cx};
     if (vb > 8192) {
         set_error("axis_histograms: at most 8192 voxel bins");
         return EINVAL_;
     }
-    const unsigned grid = (unsigned)std::min<int64_t>(w.rows, 4096);
-    hipLaunchKernelGGL(k_axis_histograms, dim3(grid), dim3(kRowThreads), (unsigned)(2 * vb * 4), s, vox, w, a,
-                       vec_ok(vox, nx), (unsigned long long *)yb, (unsigned long long *)zb);
-    MXB_FHIP(hipGetLastError());
     const int cols = 8 * vb * 4 <= kXrLds ? 8 : 4;  // vb <= 8192: 4 columns fit
     const int64_t blocks = ((nx + cols - 1) / cols) * ((nz + kXrThreads - 1) / kXrThreads);
     if (blocks > 0x7fffffffLL) {
@@ -658,13 +663,22 @@ static int device_axis_histograms(const uint16_t *vox, int64_t nz, int64_t ny, i
     const void *fn = cols == 8 ? reinterpret_cast<const void *>(&k_axis_xr<8>)
                                : reinterpret_cast<const void *>(&k_axis_xr<4>);
     MXB_FHIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, smem));
+    std::vector<int32_t> host(65536);
+    voxel_lut(vmin, vmax, vb, host.data());
+    int32_t *d = (int32_t *)ws;
+    MXB_FHIP(hipMemcpyAsync(d, host.data(), host.size() * 4, hipMemcpyHostToDevice, s));
+    AxisHistArgs a{nz, ny, nx, z_offset, Nz, Nr, vb, d, (int64_t)cy, (int64_t)cx};
+    const unsigned grid = (unsigned)std::min<int64_t>(w.rows, 4096);
+    hipLaunchKernelGGL(k_axis_histograms, dim3(grid), dim3(kRowThreads), (unsigned)(2 * vb * 4), s, vox, w, a,
+                       vec_ok(vox, nx), (unsigned long long *)yb, (unsigned long long *)zb);
+    MXB_FHIP_AFTER_COPY(hipGetLastError(), s);
     if (cols == 8)
         hipLaunchKernelGGL(k_axis_xr<8>, dim3((unsigned)blocks), dim3(kXrThreads), (unsigned)smem, s, vox, a,
                            vec_ok(vox, nx), (unsigned long long *)xb, (unsigned long long *)rb);
     else
         hipLaunchKernelGGL(k_axis_xr<4>, dim3((unsigned)blocks), dim3(kXrThreads), (unsigned)smem, s, vox, a,
                            vec_ok(vox, nx), (unsigned long long *)xb, (unsigned long long *)rb);
-    MXB_FHIP(hipGetLastError());
+    MXB_FHIP_AFTER_COPY(hipGetLastError(), s);
     MXB_FHIP(hipStreamSynchronize(s));
     return OK;
 }

@@ -86,6 +86,8 @@ hipError_t launch_line_pass_f32(const float *in, const LineGeom &g, const Epilog
                                 const float *taps, int R, Arith arith, hipStream_t s);
 hipError_t launch_init_state(const uint8_t *mask, float *u, int64_t n, hipStream_t s);
 hipError_t launch_quantise(const float *u, uint16_t *q, float *f, int64_t n, hipStream_t s);
+// m = (u == 1.0f): the reference on-disk path's per-chunk mask rule.
+hipError_t launch_mask_from_state(const float *u, uint8_t *m, int64_t n, hipStream_t s);
 
 // LDS-staged FP32 MFMA pass (mxb_kernels_f32_lds.hip): 16-byte loads of whole
 // rows; needs 16-byte-aligned input rows.

@@ -332,6 +332,15 @@ __global__ __launch_bounds__(256) void k_quantise(const float *__restrict__ u,
     }
 }
 
+// The reference on-disk path's mask rule (store_mask, cpu:331-337): a voxel
+// is illuminated when its state is exactly 1.0f.
+__global__ __launch_bounds__(256) void k_mask_from_state(const float *__restrict__ u,
+                                                         uint8_t *__restrict__ m, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        m[i] = u[i] == 1.0f ? 1 : 0;
+}
+
 // ---------------------------------------------------------------- launchers
 static inline int64_t grid_stride_blocks(int64_t n) {
     int64_t b = (n + 255) / 256;
@@ -341,6 +350,12 @@ static inline int64_t grid_stride_blocks(int64_t n) {
 hipError_t launch_init_state(const uint8_t *mask, float *u, int64_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_init_state, dim3((unsigned)grid_stride_blocks(n)), dim3(256), 0, s, mask, u, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_mask_from_state(const float *u, uint8_t *m, int64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mask_from_state, dim3((unsigned)grid_stride_blocks(n)), dim3(256), 0, s, u, m, n);
     return hipGetLastError();
 }
 

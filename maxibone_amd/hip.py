@@ -26,6 +26,7 @@ def rt():
         H.hipFree.argtypes = [vp]
         H.hipMemcpy.argtypes = [vp, vp, sz, ctypes.c_int]
         H.hipMemset.argtypes = [vp, ctypes.c_int, sz]
+        H.hipMemcpy2D.argtypes = [vp, sz, vp, sz, sz, sz, ctypes.c_int]
         H.hipDeviceSynchronize.argtypes = []
         H.hipStreamCreate.argtypes = [ctypes.POINTER(vp)]
         H.hipStreamDestroy.argtypes = [vp]
@@ -67,6 +68,24 @@ class DeviceBuffer:
     def to_array(self, dtype, shape):
         out = np.empty(shape, dtype)
         check(rt().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, D2H))
+        return out
+
+    def box(self, dtype, shape, origin, size):
+        """Copy the box [origin, origin + size) of the (nz, ny, nx) C-order
+        volume of `dtype` held in this buffer to a new host array."""
+        dtype = np.dtype(dtype)
+        nz, ny, nx = (int(v) for v in shape)
+        z0, y0, x0 = (int(v) for v in origin)
+        bz, by, bx = (int(v) for v in size)
+        if not (0 <= z0 and z0 + bz <= nz and 0 <= y0 and y0 + by <= ny and 0 <= x0 and x0 + bx <= nx):
+            raise ValueError(f"box {origin}+{size} outside the volume {shape}")
+        if nz * ny * nx * dtype.itemsize > self.nbytes:
+            raise ValueError("volume larger than the buffer")
+        out = np.empty((bz, by, bx), dtype)
+        e = dtype.itemsize
+        for z in range(bz):
+            src = self.ptr + (((z0 + z) * ny + y0) * nx + x0) * e
+            check(rt().hipMemcpy2D(out[z].ctypes.data, bx * e, src, nx * e, bx * e, by, D2H))
         return out
 
     def fill_bytes(self, value):

@@ -43,3 +43,19 @@ def gpu():
     if capi.device_count() < 1:
         pytest.skip("no HIP device visible")
     return capi
+
+
+@pytest.fixture(scope="session")
+def workload_mask():
+    """The named workload's mask, built once per session (the 2048x2048x1536
+    mask takes ~30 s and 6.4 GB)."""
+    from maxibone_amd.synth import WORKLOADS
+    cache = {}
+
+    def get(name):
+        if name not in cache:
+            print(f"building the {name} mask", flush=True)
+            cache[name] = WORKLOADS[name].mask()
+        return cache[name]
+
+    return get

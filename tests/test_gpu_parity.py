@@ -446,3 +446,138 @@ def test_batched_random_shapes_equal_single_volume(gpu, prec):
     for m, k, q, f in zip(masks, kernels, qs, fs):
         q1, f1 = gpu.diffusion(m, k, 3, precision=prec, u16=True, f32=True)
         assert np.array_equal(f, f1) and np.array_equal(q, q1)
+
+
+# ---------------------------------------------------------------- streaming paths
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("chunk_z,temporal", [(7, 1), (16, 3), (40, 2), (5, 4)])
+def test_chunked_16bit_equals_in_memory(gpu, prec, chunk_z, temporal):
+    """16-bit chunked runs keep 16-bit host state between chunk visits and run
+    the last repetition in FP32: bit-equal to the in-memory 16-bit form."""
+    rng = np.random.default_rng(chunk_z * 31 + temporal)
+    m = (rng.random((77, 33, 70)) > 0.97).astype(np.uint8)
+    k = gauss_kernel(2.2).astype(np.float32)
+    for reps in (1, 2, 6):
+        _say("chunked 16-bit", prec, chunk_z, temporal, reps)
+        q0, u0 = gpu.diffusion(m, k, reps, precision=prec, f32=True)
+        q1, u1 = gpu.diffusion_chunked(m, k, reps, chunk_z, temporal, precision=prec, f32=True)
+        assert np.array_equal(u0.view(np.uint32), u1.view(np.uint32)) and np.array_equal(q0, q1)
+
+
+@pytest.fixture
+def device_budget():
+    """Set MXB_DEVICE_BUDGET (read by the library at every call); restores it."""
+    old = os.environ.get("MXB_DEVICE_BUDGET")
+
+    def use(nbytes):
+        if nbytes is None:
+            os.environ.pop("MXB_DEVICE_BUDGET", None)
+        else:
+            os.environ["MXB_DEVICE_BUDGET"] = str(int(nbytes))
+
+    yield use
+    use(old)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp32_exact"])
+def test_drop_in_streams_when_the_volume_exceeds_the_budget(gpu, device_budget, prec):
+    """A volume larger than the device budget takes the z-chunk streaming path
+    (the reference switches to its out-of-core form, pyb:40-45): results are
+    bit-equal to the in-HBM form, for the drop-in module and the C ABI."""
+    from maxibone_amd.diffusion import diffusion
+    rng = np.random.default_rng(404)
+    m = (rng.random((96, 48, 64)) > 0.96).astype(np.uint8)
+    k = gauss_kernel(1.5).astype(np.float32)
+    reps = 5
+    device_budget(None)
+    q0, u0 = gpu.diffusion(m, k, reps, precision=prec, f32=True)
+    # room for about 40 layers of mask, state and output: chunks of >= 16 layers
+    device_budget(40 * m.shape[1] * m.shape[2] * 11)
+    _say("budget fallback", prec)
+    q1, u1 = gpu.diffusion(m, k, reps, precision=prec, f32=True, verbose=1)
+    assert np.array_equal(u0.view(np.uint32), u1.view(np.uint32)) and np.array_equal(q0, q1)
+    if prec == "fp32":
+        out = np.zeros(m.shape, np.uint16)
+        diffusion(m, k, out, reps)
+        assert np.array_equal(out, q0)
+    device_budget(None)
+
+
+def _cavity_mask():
+    """Solid block with small cavities: cavity voxels approach 1 and reach
+    exactly 1.0f, where the reference's (u == 1.0f) rule and the true mask
+    can part ways."""
+    shape = (53, 24, 40)
+    m = np.zeros(shape, np.uint8)
+    m[8:45, 2:22, 3:37] = 1
+    rng = np.random.default_rng(5)
+    for _ in range(30):
+        a = rng.integers(1, 5, 3)
+        o = [rng.integers(10, 43 - a[0]), rng.integers(4, 20 - a[1]), rng.integers(5, 35 - a[2])]
+        m[o[0]:o[0] + a[0], o[1]:o[1] + a[1], o[2]:o[2] + a[2]] = 0
+    return m
+
+
+def test_on_disk_reference_mask_rule(gpu, tmp_path):
+    """The opt-in reference on-disk semantics: the mask is re-derived as
+    (u == 1.0f) at every chunk visit (cpu:331-337, :445).  With the reference
+    CPU arithmetic the uint16 file equals the oracle's quirk mode (itself
+    pinned to the reference's own on-disk build) bit for bit; the MFMA form
+    is within 1 LSB."""
+    m = _cavity_mask()
+    shape = m.shape
+    k = gauss_kernel(1.1).astype(np.float32)
+    fin = str(tmp_path / "mask.uint8")
+    m.tofile(fin)
+    for chunk_z in (9, 16, 53):
+        for reps in (1, 10, 20):
+            u_ref = oracle.diffusion_chunked_f32(m, k, reps, chunk_z, quirk_mask=True)
+            q_ref = oracle.quantise_u16(u_ref)
+            for prec in ("fp32_exact", "fp32"):
+                fout = str(tmp_path / f"out_{chunk_z}_{reps}_{prec}.uint16")
+                _say("reference mask rule", chunk_z, reps, prec)
+                gpu.diffusion_on_disk(fin, k, fout, shape, chunk_z, reps, precision=prec, reference_mask=True)
+                got = np.fromfile(fout, np.uint16).reshape(shape)
+                if prec == "fp32_exact":
+                    assert np.array_equal(got, q_ref)
+                else:
+                    assert _u16_close(got, q_ref)
+    # The Gaussian case leaves the two rules within a float ulp of each other.
+    # Dyadic taps with a negative centre (values stay dyadic, so every form
+    # computes them exactly) make a voxel pass through exactly 1.0f and move on
+    # under the true mask, while the reference's rule pins it at 1: the rules
+    # then differ by whole uint16 ranges.  Compared where both results lie in
+    # [0, 1] (elsewhere uint16 conversion wraps).
+    taps = np.array([1.0, -0.5, 0.5], np.float32)
+    for chunk_z, reps in ((9, 2), (16, 3)):
+        u_ref = oracle.diffusion_chunked_f32(m, taps, reps, chunk_z, quirk_mask=True)
+        u_true = oracle.diffusion_chunked_f32(m, taps, reps, chunk_z)
+        ok = (u_ref >= 0) & (u_ref <= 1) & (u_true >= 0) & (u_true <= 1)
+        q_ref, q_true = oracle.quantise_u16(u_ref), oracle.quantise_u16(u_true)
+        assert np.count_nonzero((q_ref != q_true) & ok) > 1000
+        for prec in ("fp32_exact", "fp32"):
+            fout = str(tmp_path / f"dyadic_{chunk_z}_{reps}_{prec}.uint16")
+            _say("reference mask rule, dyadic taps", chunk_z, reps, prec)
+            gpu.diffusion_on_disk(fin, taps, fout, shape, chunk_z, reps, precision=prec, reference_mask=True)
+            assert np.array_equal(np.fromfile(fout, np.uint16).reshape(shape)[ok], q_ref[ok])
+            gpu.diffusion_on_disk(fin, taps, fout, shape, chunk_z, reps, precision=prec)
+            assert np.array_equal(np.fromfile(fout, np.uint16).reshape(shape)[ok], q_true[ok])
+    with pytest.raises(ValueError):
+        gpu.diffusion_on_disk(fin, k, str(tmp_path / "x.uint16"), shape, 9, 2, precision="bf16",
+                              reference_mask=True)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp32_exact"])
+def test_on_disk_precisions(gpu, tmp_path, prec):
+    """The file-backed form with any precision equals the in-memory form."""
+    rng = np.random.default_rng(9)
+    shape = (70, 20, 36)
+    m = (rng.random(shape) > 0.95).astype(np.uint8)
+    k = gauss_kernel(1.8).astype(np.float32)
+    fin, fout = str(tmp_path / "mask.uint8"), str(tmp_path / "out.uint16")
+    m.tofile(fin)
+    _say("on-disk", prec)
+    gpu.diffusion_on_disk(fin, k, fout, shape, 12, 5, precision=prec)
+    got = np.fromfile(fout, np.uint16).reshape(shape)
+    want = gpu.diffusion(m, k, 5, precision=prec)[0]
+    assert np.array_equal(got, want)
