@@ -176,6 +176,27 @@ int mxb_device_diffusion(const uint8_t *dev_mask, int64_t nz, int64_t ny, int64_
                          int precision, void *dev_state0, void *dev_state1,
                          uint16_t *dev_out_u16, float *dev_out_f32, void *stream);
 
+/* Bytes of device memory that mxb_device_diffusion_noalloc holds for its state
+ * buffers on every device it runs on (20 MiB). */
+#define MXB_NOALLOC_SCRATCH_BYTES 20971520
+
+/* mxb_device_diffusion without the state buffers: no reference counterpart.
+ * It is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The two
+ * state buffers live in a fixed device array of MXB_NOALLOC_SCRATCH_BYTES per
+ * device, held until the process ends, so the volume's two state buffers
+ * (2 * mxb_state_bytes, the second rounded up to 256 bytes) must fit in it:
+ * every volume of at most 128^3 voxels with an x extent of at least 32 does.
+ * Larger volumes fail with MXB_EINVAL before any device work.  Allocates no
+ * device or host memory and records no timing events.  Every call on a device
+ * uses that device's one array: calls that overlap on one device (other
+ * threads, other streams) race; calls ordered on one stream do not.
+ * Asynchronous, like mxb_device_diffusion. */
+int mxb_device_diffusion_noalloc(const uint8_t *dev_mask, int64_t nz, int64_t ny, int64_t nx,
+                                 const float *kernel, int64_t kernel_size, int64_t repetitions,
+                                 int precision, uint16_t *dev_out_u16, float *dev_out_f32,
+                                 void *stream);
+
 /* ---- per-kernel timing (HIP events on the launch stream) ------------------ */
 
 /* Kernel classes reported by mxb_kernel_times. */

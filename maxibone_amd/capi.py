@@ -18,6 +18,7 @@ HEADERS = [os.path.join(os.path.dirname(PKG), "include", h)
 OK, EINVAL, EDEVICE, ENOMEM, EIO = 0, 1, 2, 3, 4
 PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "fp32_fma": 3, "fp32_exact": 4}
 K_ZPASS, K_YPASS, K_XPASS, K_OTHER = 0, 1, 2, 3
+NOALLOC_SCRATCH_BYTES = 20 << 20  # MXB_NOALLOC_SCRATCH_BYTES
 
 _i64 = ctypes.c_int64
 _vp = ctypes.c_void_p
@@ -55,6 +56,8 @@ def lib():
                                       ctypes.c_int],
             "mxb_device_diffusion": [_vp, _i64, _i64, _i64, _vp, _i64, _i64, ctypes.c_int, _vp, _vp,
                                      _vp, _vp, _vp],
+            "mxb_device_diffusion_noalloc": [_vp, _i64, _i64, _i64, _vp, _i64, _i64, ctypes.c_int, _vp,
+                                             _vp, _vp],
             "mxb_set_kernel_timing": [ctypes.c_int],
             "mxb_kernel_times": [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(_i64)],

@@ -59,10 +59,13 @@ static hipEvent_t take_event() {
     return e;
 }
 
+// Set while a call must not create events (the no-allocation entry point).
+static thread_local bool g_timing_off = false;
+
 // Runs `fn` (a launch on stream s), bracketing it with events when timing is on.
 template <typename F>
 static hipError_t timed(int cls, hipStream_t s, F &&fn) {
-    if (!g_timing.enabled) return fn();
+    if (!g_timing.enabled || g_timing_off) return fn();
     std::lock_guard<std::mutex> lk(g_timing.mu);
     hipEvent_t a = take_event(), b = take_event();
     if (a == nullptr || b == nullptr) return fn();
@@ -688,6 +691,47 @@ static int map_write(const char *path, size_t len, MappedFile &m, bool unlink_af
     return OK;
 }
 
+// ---------------------------------------------------------------- no-allocation entry
+// The two state buffers of mxb_device_diffusion_noalloc: a file-scope device
+// array that every device the library runs on holds until the process ends.
+// In host code `scratch` only names the symbol; its address on a device comes
+// from hipGetSymbolAddress with that device current, looked up once per
+// device and kept in g_scratch_ptr.
+static __device__ __align__(256) unsigned char scratch[MXB_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+struct TimingOff {
+    TimingOff() { g_timing_off = true; }
+    ~TimingOff() { g_timing_off = false; }
+};
+}  // namespace
+
+// Device address of `scratch` on the current device.
+static int scratch_address(void **out) {
+    int dev = 0;
+    MXB_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device ordinal " + std::to_string(dev) + " beyond the supported " + std::to_string(kMaxDevices));
+        return EINVAL_;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        MXB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (p == nullptr) {
+            set_error("hipGetSymbolAddress returned no address for the scratch array");
+            return EDEVICE;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return OK;
+}
+
 }  // namespace mxb
 
 using namespace mxb;
@@ -852,6 +896,40 @@ int mxb_device_diffusion(const uint8_t *dev_mask, int64_t nz, int64_t ny, int64_
     }
     return run_steps(dev_mask, nz, ny, nx, kernel, kernel_size, repetitions, precision, dev_state0,
                      dev_state1, dev_out_u16, dev_out_f32, (hipStream_t)stream);
+}
+
+int mxb_device_diffusion_noalloc(const uint8_t *dev_mask, int64_t nz, int64_t ny, int64_t nx,
+                                 const float *kernel, int64_t kernel_size, int64_t repetitions,
+                                 int precision, uint16_t *dev_out_u16, float *dev_out_f32,
+                                 void *stream) {
+    int rc = check_common(nz, ny, nx, kernel, kernel_size, repetitions, precision);
+    if (rc != OK) return rc;
+    if (dev_out_u16 == nullptr && dev_out_f32 == nullptr) {
+        set_error("no output buffer given");
+        return EINVAL_;
+    }
+    if (nz * ny * nx == 0) return OK;
+    if (dev_mask == nullptr) {
+        set_error("device mask must not be null");
+        return EINVAL_;
+    }
+    // two state buffers, the second at the next 256-byte boundary
+    const int64_t sb = state_bytes(precision, nz, ny, nx);
+    const int64_t off = (sb + 255) / 256 * 256;
+    if (off + sb > (int64_t)MXB_NOALLOC_SCRATCH_BYTES) {
+        set_error("volume needs " + std::to_string(off + sb) + " bytes of state; the no-allocation entry "
+                  "holds at most " + std::to_string((int64_t)MXB_NOALLOC_SCRATCH_BYTES));
+        return EINVAL_;
+    }
+    void *base = nullptr;
+    rc = scratch_address(&base);
+    if (rc != OK) return rc;
+    // Every byte the kernels read from the state buffers is written earlier in
+    // the same call (initial state, packed mask bits, pass outputs), so
+    // nothing needs clearing between calls.
+    TimingOff no_events;
+    return run_steps(dev_mask, nz, ny, nx, kernel, kernel_size, repetitions, precision, base,
+                     static_cast<char *>(base) + off, dev_out_u16, dev_out_f32, (hipStream_t)stream);
 }
 
 int mxb_device_count(void) {

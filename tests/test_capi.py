@@ -132,3 +132,10 @@ def test_edt_argument_checks_before_the_device():
     assert L.mxb_edt_field(big.ctypes.data, 1, E.MAX_AXIS + 1, 1, None, None, 0) == capi.EINVAL
     with pytest.raises(ValueError):
         E.edt(np.zeros((4, 4), bool))
+
+
+def test_header_states_the_noalloc_scratch_size():
+    import re
+    text = open(capi.HEADERS[0]).read()
+    m = re.search(r"#define MXB_NOALLOC_SCRATCH_BYTES (\d+)", text)
+    assert m and int(m.group(1)) == capi.NOALLOC_SCRATCH_BYTES
