@@ -41,7 +41,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "batch", "fields", "edt"])
     p.add_argument("--precision", default="fp32",
-                   choices=["fp32", "bf16", "fp16", "fp32_fma", "fp32_exact"])
+                   choices=["fp32", "bf16", "fp16", "fp32_fma", "fp32_exact", "fp32_split"])
     p.add_argument("--reps", type=int, default=None, help="override the workload's repetitions")
     p.add_argument("--sigma", type=float, default=None,
                    help="override the workload's sigma (same mask; R = round(4 sigma))")
@@ -81,7 +81,8 @@ def algorithmic(workload, precision):
     the mask, packed to one bit per voxel once per diffusion."""
     sb = 2 if precision in ("bf16", "fp16") else 4
     K = 2 * workload.radius + 1
-    return {"zpass": (2 * sb, 2 * K), "ypass": (2 * sb, 2 * K), "xpass": (2 * sb + 0.125, 2 * K)}
+    f = 2 * K * (3 if precision == "fp32_split" else 1)  # the split form: three FP16 products per tap
+    return {"zpass": (2 * sb, f), "ypass": (2 * sb, f), "xpass": (2 * sb + 0.125, f)}
 
 
 def cpu_baseline(workload, target_s):
@@ -579,7 +580,8 @@ def device_run(args, w, mask_h, k, reps, precision):
                             "GBps": b * n / avg_s / 1e9, "TFLOPs": f * n / avg_s / 1e12}
     dom = max(per_kernel, key=lambda kk: per_kernel[kk]["avg_ms"])
     d = per_kernel[dom]
-    mfma_peak = PEAK_MFMA.get(precision.split("_")[0], PEAK_MFMA["fp32"])
+    mfma_peak = PEAK_MFMA["fp16"] if precision == "fp32_split" else PEAK_MFMA.get(precision.split("_")[0],
+                                                                                    PEAK_MFMA["fp32"])
     compute_peak = PEAK_VALU_FP32 if precision in ("fp32_fma", "fp32_exact") else mfma_peak
     frac_hbm = d["GBps"] / PEAK_HBM_GBS
     frac_cmp = d["TFLOPs"] / compute_peak

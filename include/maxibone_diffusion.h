@@ -37,8 +37,17 @@ typedef enum {
     MXB_BF16 = 1,       /* BF16 state, BF16 MFMA passes, FP32 final repetition    */
     MXB_FP16 = 2,       /* FP16 state, FP16 MFMA passes, FP32 final repetition    */
     MXB_FP32_FMA = 3,   /* FP32 state, one VALU FMA per tap                       */
-    MXB_FP32_EXACT = 4  /* FP32 state, product-then-add: bit-equal to the
+    MXB_FP32_EXACT = 4, /* FP32 state, product-then-add: bit-equal to the
                            reference CPU build                                  */
+    MXB_FP32_SPLIT = 5  /* FP32-class state held as FP16 hi + lo (u = hi + lo,
+                           22 significant bits); each pass runs three FP16
+                           MFMAs per chunk (hi*k_hi + hi*k_lo + lo*k_hi) on
+                           power-of-two-scaled taps with FP32 accumulation.
+                           Same HBM bytes as FP32, a fraction of its MFMA
+                           time; within the FP32 bar (1e-5, 1 uint16 LSB) of
+                           the reference CPU path.  Volumes whose x extent
+                           is not a multiple of 8 (with radius above 48:
+                           x or y extent) run MXB_FP32.                    */
 } mxb_precision;
 
 /* ---- drop-in entry points (host buffers) ------------------------------- */

@@ -16,7 +16,7 @@ HEADERS = [os.path.join(os.path.dirname(PKG), "include", h)
            for h in ("maxibone_diffusion.h", "maxibone_fields.h", "maxibone_edt.h")]
 
 OK, EINVAL, EDEVICE, ENOMEM, EIO = 0, 1, 2, 3, 4
-PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "fp32_fma": 3, "fp32_exact": 4}
+PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "fp32_fma": 3, "fp32_exact": 4, "fp32_split": 5}
 K_ZPASS, K_YPASS, K_XPASS, K_OTHER = 0, 1, 2, 3
 NOALLOC_SCRATCH_BYTES = 20 << 20  # MXB_NOALLOC_SCRATCH_BYTES
 

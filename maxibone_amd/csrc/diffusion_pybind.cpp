@@ -61,7 +61,8 @@ int precision_of(const std::string &p) {
     if (p == "fp16") return MXB_FP16;
     if (p == "fp32_fma") return MXB_FP32_FMA;
     if (p == "fp32_exact") return MXB_FP32_EXACT;
-    throw py::value_error("precision must be one of fp32, bf16, fp16, fp32_fma, fp32_exact");
+    if (p == "fp32_split") return MXB_FP32_SPLIT;
+    throw py::value_error("precision must be one of fp32, bf16, fp16, fp32_fma, fp32_exact, fp32_split");
 }
 
 // Overload 1 (pyb:27-53, bound at pyb:84).

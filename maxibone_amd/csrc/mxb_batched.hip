@@ -685,7 +685,7 @@ int mxb_batch_create(const mxb_device_volume *vols, int n, int64_t repetitions, 
         set_error("mxb_batch_create: invalid arguments");
         return EINVAL_;
     }
-    if (precision < MXB_FP32 || precision > MXB_FP32_EXACT) {
+    if (precision < MXB_FP32 || precision > MXB_FP32_SPLIT) {
         set_error("unknown precision");
         return EINVAL_;
     }
@@ -847,13 +847,14 @@ int mxb_batch_run(mxb_batch_plan *P, int mode) {
         MXB_BHIP(hipGraphLaunch(P->exec, P->stream));
         return OK;
     }
-    // per-volume launches (also the form used for non-FP32 precisions)
+    // per-volume launches (also the form used for the other precisions), with
+    // the persistent passes' three-pass structure
     for (int v = 0; v < P->n; ++v) {
         const mxb_device_volume &V = P->vols[v];
         const VolLayout &L = P->lay[v];
-        int rc = mxb_device_diffusion(V.dev_mask, V.nz, V.ny, V.nx, V.kernel, V.kernel_size, P->reps,
-                                      P->precision, P->st[0] + L.st_off, P->st[1] + L.st_off, V.dev_out_u16,
-                                      V.dev_out_f32, P->stream);
+        int rc = device_diffusion(V.dev_mask, V.nz, V.ny, V.nx, V.kernel, V.kernel_size, P->reps,
+                                  P->precision, P->st[0] + L.st_off, P->st[1] + L.st_off, V.dev_out_u16,
+                                  V.dev_out_f32, P->stream, false);
         if (rc != OK) return rc;
     }
     return OK;

@@ -109,7 +109,7 @@ static int check_common(int64_t nz, int64_t ny, int64_t nx, const float *kernel,
         set_error("repetitions must be non-negative");
         return EINVAL_;
     }
-    if (precision < MXB_FP32 || precision > MXB_FP32_EXACT) {
+    if (precision < MXB_FP32 || precision > MXB_FP32_SPLIT) {
         set_error("unknown precision " + std::to_string(precision));
         return EINVAL_;
     }
@@ -132,9 +132,12 @@ static bool is_lowp(int precision) { return precision == MXB_BF16 || precision =
 
 // Every form needs two FP32-sized state buffers: the 16-bit forms use the
 // first half of each for their repetitions and the whole of both for the
-// final FP32 repetition.  The first buffer also carries the mask packed to
-// bits after the state (at byte state_f32_bytes).
-static int64_t state_f32_bytes(int64_t n) { return (n * 4 + 255) / 256 * 256; }
+// final FP32 repetition; the split form holds its FP16 high plane at the
+// start and its low plane split_lo_off(n) elements further.  The first
+// buffer also carries the mask packed to bits after the state (at byte
+// state_f32_bytes).
+static int64_t split_lo_off(int64_t n) { return (n + 127) / 128 * 128; }
+static int64_t state_f32_bytes(int64_t n) { return split_lo_off(n) * 4; }
 
 int64_t state_bytes(int precision, int64_t nz, int64_t ny, int64_t nx) {
     (void)precision;
@@ -204,15 +207,24 @@ static int run_steps_f32(const uint8_t *d_mask, const MaskBits &bits, int64_t nz
 static int run_lowp_core(const uint8_t *d_mask, const MaskBits &bits, int64_t nz, int64_t ny,
                          int64_t nx, const float *k, int64_t K, int64_t n16, bool final_f32, bool bf,
                          bool init_from_mask, void *s0, void *s1, uint16_t *out_u16, float *out_f32,
-                         float **final_state, uint16_t **final16, hipStream_t s) {
+                         float **final_state, uint16_t **final16, bool fused, hipStream_t s) {
     const int64_t n = nz * ny * nx;
     const int R = (int)(K / 2);
     if (n == 0) return OK;
     uint16_t *cur = (uint16_t *)s0, *tmp = (uint16_t *)s1;
     if (init_from_mask) MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_init_lowp(d_mask, cur, n, bf, s); }));
     const LineGeom gz = geom_z(nz, ny, nx), gy = geom_y(nz, ny, nx), gx = geom_x(nz, ny, nx);
+    const int prec = bf ? MXB_BF16 : MXB_FP16;
+    fused = fused && xy_fusion_enabled() && xy_fused_supported(tmp, cur, bits, nz, ny, nx, R, prec, SplitIO{});
     for (int64_t t = 0; t < n16; ++t) {
         MXB_HIP(timed(MXB_K_ZPASS, s, [&] { return launch_line_pass_lowp(cur, tmp, nullptr, gz, k, R, bf, s); }));
+        if (fused) {
+            // x then y in one pass, back into cur, illuminating
+            MXB_HIP(timed(MXB_K_XPASS, s, [&] {
+                return launch_xy_fused(tmp, cur, bits, nz, ny, nx, k, R, prec, SplitIO{}, s);
+            }));
+            continue;
+        }
         MXB_HIP(timed(MXB_K_YPASS, s, [&] { return launch_line_pass_lowp(tmp, cur, nullptr, gy, k, R, bf, s); }));
         MXB_HIP(timed(MXB_K_XPASS, s, [&] { return launch_line_pass_lowp(cur, tmp, d_mask, gx, k, R, bf, s, bits); }));
         std::swap(cur, tmp);
@@ -233,17 +245,96 @@ static int run_lowp_core(const uint8_t *d_mask, const MaskBits &bits, int64_t nz
 // the last in FP32.
 static int run_steps_lowp(const uint8_t *d_mask, const MaskBits &bits, int64_t nz, int64_t ny,
                           int64_t nx, const float *k, int64_t K, int64_t reps, bool bf, void *s0,
-                          void *s1, uint16_t *out_u16, float *out_f32, hipStream_t s) {
+                          void *s1, uint16_t *out_u16, float *out_f32, bool fused, hipStream_t s) {
     if (reps <= 1)
         return run_steps_f32(d_mask, bits, nz, ny, nx, k, K, reps, Arith::MFMA, (float *)s0,
                              (float *)s1, true, out_u16, out_f32, nullptr, s);
     return run_lowp_core(d_mask, bits, nz, ny, nx, k, K, reps - 1, true, bf, true, s0, s1, out_u16,
-                         out_f32, nullptr, nullptr, s);
+                         out_f32, nullptr, nullptr, fused, s);
+}
+
+// How the split form serves a volume: 0 not at all (it runs as FP32), 1 with
+// the three-pass form, 2 with the z-pass and the fused x -> y pass.
+static int split_serves(int64_t nz, int64_t ny, int64_t nx, int R, const void *s0, const void *s1,
+                        const MaskBits &bits, uint16_t *out_u16, float *out_f32, bool fused) {
+    if (R > kMaxRadiusLowp || nz * ny * nx == 0) return 0;
+    const int64_t lo = split_lo_off(nz * ny * nx);
+    SplitIO io;
+    io.in_lo = io.out_lo = lo;
+    MaskBits b = bits;
+    if (b.w == nullptr) b.w = reinterpret_cast<const uint32_t *>(s0);  // the passes will read bits
+    const uint16_t *a = (const uint16_t *)s0, *c = (const uint16_t *)s1;
+    const uint8_t *m = reinterpret_cast<const uint8_t *>(s0);
+    SplitIO fin = io;  // the last pass writes the outputs, or the FP32 state into a state buffer
+    fin.fin_u16 = out_u16;
+    fin.fin_f32 = out_f32 != nullptr ? out_f32 : reinterpret_cast<float *>(const_cast<void *>(s0));
+    if (!split_pass_supported(a, c, nullptr, b, geom_z(nz, ny, nx), R, io)) return 0;
+    if (fused && xy_fusion_enabled() && xy_fused_supported(c, a, b, nz, ny, nx, R, MXB_FP32_SPLIT, io) &&
+        xy_fused_supported(c, a, b, nz, ny, nx, R, MXB_FP32_SPLIT, fin))
+        return 2;
+    return split_pass_supported(c, a, nullptr, b, geom_y(nz, ny, nx), R, io) &&
+                   split_pass_supported(a, c, m, b, geom_x(nz, ny, nx), R, io) &&
+                   split_pass_supported(a, c, m, b, geom_x(nz, ny, nx), R, fin)
+               ? 1
+               : 0;
+}
+
+// Split form: reps repetitions on FP16 hi + lo planes, starting from the mask
+// (from_f32 false) or from the FP32 state in s0 (split into s1 first).  Per
+// repetition the z-pass writes the other buffer; then either the fused
+// x -> y pass writes back (form 2) or the y-pass (in (z, x, y) order) and the
+// x-pass do, swapping the buffers (form 1).  The last pass writes out_u16 /
+// out_f32; with out_f32 null and final_state non-null it writes the FP32
+// state into the buffer its input is not in and reports it.
+static int run_steps_split(const uint8_t *d_mask, const MaskBits &bits, int64_t nz, int64_t ny, int64_t nx,
+                           const float *k, int64_t K, int64_t reps, void *s0, void *s1, bool from_f32,
+                           uint16_t *out_u16, float *out_f32, float **final_state, int form, hipStream_t s) {
+    const int64_t n = nz * ny * nx;
+    const int R = (int)(K / 2);
+    if (n == 0) return OK;
+    if (reps == 0)
+        return run_steps_f32(d_mask, bits, nz, ny, nx, k, K, 0, Arith::MFMA, (float *)s0, (float *)s1,
+                             !from_f32, out_u16, out_f32, final_state, s);
+    const int64_t lo = split_lo_off(n);
+    uint16_t *cur = (uint16_t *)s0, *tmp = (uint16_t *)s1;
+    if (from_f32) {
+        MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_split_state((const float *)s0, tmp, lo, n, s); }));
+        std::swap(cur, tmp);
+    } else {
+        MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_init_split(d_mask, cur, lo, n, s); }));
+    }
+    const LineGeom gz = geom_z(nz, ny, nx), gy = geom_y(nz, ny, nx), gx = geom_x(nz, ny, nx);
+    SplitIO io;
+    io.in_lo = io.out_lo = lo;
+    for (int64_t t = 0; t < reps; ++t) {
+        const bool last = t == reps - 1;
+        MXB_HIP(timed(MXB_K_ZPASS, s, [&] { return launch_line_pass_split(cur, tmp, nullptr, bits, gz, k, R, io, s); }));
+        // the last pass's input is tmp (fused) or cur (three-pass): its other
+        // buffer takes the FP32 state when one is wanted
+        uint16_t *const free_buf = form == 2 ? cur : tmp;
+        SplitIO ex = io;
+        if (last) {
+            ex.fin_u16 = out_u16;
+            ex.fin_f32 = out_f32 != nullptr ? out_f32 : (final_state != nullptr ? (float *)free_buf : nullptr);
+            if (ex.fin_f32 == nullptr && ex.fin_u16 == nullptr) ex.fin_f32 = (float *)free_buf;
+            if (final_state) *final_state = ex.fin_f32;
+        }
+        if (form == 2) {
+            MXB_HIP(timed(MXB_K_XPASS, s, [&] {
+                return launch_xy_fused(tmp, cur, bits, nz, ny, nx, k, R, MXB_FP32_SPLIT, ex, s);
+            }));
+            continue;
+        }
+        MXB_HIP(timed(MXB_K_YPASS, s, [&] { return launch_line_pass_split(tmp, cur, nullptr, bits, gy, k, R, io, s); }));
+        MXB_HIP(timed(MXB_K_XPASS, s, [&] { return launch_line_pass_split(cur, tmp, d_mask, bits, gx, k, R, ex, s); }));
+        std::swap(cur, tmp);
+    }
+    return OK;
 }
 
 static int run_steps(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t nx, const float *k,
                      int64_t K, int64_t reps, int precision, void *s0, void *s1,
-                     uint16_t *out_u16, float *out_f32, hipStream_t s) {
+                     uint16_t *out_u16, float *out_f32, hipStream_t s, bool fused = true) {
     MaskBits bits;
     if (reps > 0 && nz * ny * nx > 0) {
         uint32_t *w = reinterpret_cast<uint32_t *>(static_cast<char *>(s0) + state_f32_bytes(nz * ny * nx));
@@ -255,7 +346,13 @@ static int run_steps(const uint8_t *d_mask, int64_t nz, int64_t ny, int64_t nx, 
             return EINVAL_;
         }
         return run_steps_lowp(d_mask, bits, nz, ny, nx, k, K, reps, precision == MXB_BF16, s0, s1,
-                              out_u16, out_f32, s);
+                              out_u16, out_f32, fused, s);
+    }
+    if (precision == MXB_FP32_SPLIT) {
+        const int form = split_serves(nz, ny, nx, (int)(K / 2), s0, s1, bits, out_u16, out_f32, fused);
+        if (form != 0)
+            return run_steps_split(d_mask, bits, nz, ny, nx, k, K, reps, s0, s1, false, out_u16, out_f32,
+                                   nullptr, form, s);
     }
     return run_steps_f32(d_mask, bits, nz, ny, nx, k, K, reps, arith_of(precision), (float *)s0,
                          (float *)s1, true, out_u16, out_f32, nullptr, s);
@@ -318,8 +415,9 @@ static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
         return EINVAL_;
     }
     const Arith arith = arith_of(lowp ? MXB_FP32 : precision);
+    const bool split = precision == MXB_FP32_SPLIT;
     const int64_t esz = lowp ? 2 : 4;  // bytes per host state element
-    const int64_t align = lowp ? 32 : 1;
+    const int64_t align = lowp || split ? 32 : 1;
     if (T < 1) T = 1;
     if (cz < 1) cz = 1;
     if (cz > nz) cz = nz;
@@ -336,7 +434,8 @@ static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
             hipError_t e;
             if ((e = hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking)) != hipSuccess) return e;
             if ((e = dm.alloc(cap)) != hipSuccess || (e = db.alloc(bits_bytes)) != hipSuccess ||
-                (e = d0.alloc(cap * 4)) != hipSuccess || (e = d1.alloc(cap * 4)) != hipSuccess)
+                (e = d0.alloc(state_f32_bytes(cap))) != hipSuccess ||
+                (e = d1.alloc(state_f32_bytes(cap))) != hipSuccess)
                 return e;
             return u16 ? dq.alloc(cap * 2) : hipSuccess;
         }
@@ -401,7 +500,12 @@ static int chunked_core(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
             int rc;
             if (lowp) {
                 rc = run_lowp_core((const uint8_t *)S.dm.p, bits, lz, ny, nx, k, K, last_block ? tb - 1 : tb,
-                                   last_block, bf, done == 0, S.d0.p, S.d1.p, q, nullptr, &fin, &fin16, cs);
+                                   last_block, bf, done == 0, S.d0.p, S.d1.p, q, nullptr, &fin, &fin16, true, cs);
+            } else if (split && split_serves(lz, ny, nx, (int)R, S.d0.p, S.d1.p, bits, q, nullptr, true) != 0) {
+                // FP32 state between visits, split into planes on the device
+                rc = run_steps_split((const uint8_t *)S.dm.p, bits, lz, ny, nx, k, K, tb, S.d0.p, S.d1.p,
+                                     done > 0, q, nullptr, &fin,
+                                     split_serves(lz, ny, nx, (int)R, S.d0.p, S.d1.p, bits, q, nullptr, true), cs);
             } else {
                 if (done == 0)
                     MXB_HIP(launch_init_state((const uint8_t *)S.dm.p, (float *)S.d0.p, lz * layer, cs));
@@ -742,7 +846,7 @@ const char *mxb_last_error(void) { return g_error.c_str(); }
 const char *mxb_version(void) { return "maxibone_amd diffusion 0.1 (gfx950)"; }
 
 int64_t mxb_state_bytes(int precision, int64_t nz, int64_t ny, int64_t nx) {
-    if (precision < MXB_FP32 || precision > MXB_FP32_EXACT || nz < 0 || ny < 0 || nx < 0) {
+    if (precision < MXB_FP32 || precision > MXB_FP32_SPLIT || nz < 0 || ny < 0 || nx < 0) {
         set_error("mxb_state_bytes: unknown precision or negative extent");
         return -1;
     }
@@ -884,6 +988,17 @@ int mxb_device_diffusion(const uint8_t *dev_mask, int64_t nz, int64_t ny, int64_
                          const float *kernel, int64_t kernel_size, int64_t repetitions,
                          int precision, void *dev_state0, void *dev_state1, uint16_t *dev_out_u16,
                          float *dev_out_f32, void *stream) {
+    return device_diffusion(dev_mask, nz, ny, nx, kernel, kernel_size, repetitions, precision, dev_state0,
+                            dev_state1, dev_out_u16, dev_out_f32, stream, true);
+}
+
+}  // extern "C"
+
+namespace mxb {
+
+int device_diffusion(const uint8_t *dev_mask, int64_t nz, int64_t ny, int64_t nx, const float *kernel,
+                     int64_t kernel_size, int64_t repetitions, int precision, void *dev_state0,
+                     void *dev_state1, uint16_t *dev_out_u16, float *dev_out_f32, void *stream, bool fused) {
     int rc = check_common(nz, ny, nx, kernel, kernel_size, repetitions, precision);
     if (rc != OK) return rc;
     if (dev_out_u16 == nullptr && dev_out_f32 == nullptr) {
@@ -895,8 +1010,12 @@ int mxb_device_diffusion(const uint8_t *dev_mask, int64_t nz, int64_t ny, int64_
         return EINVAL_;
     }
     return run_steps(dev_mask, nz, ny, nx, kernel, kernel_size, repetitions, precision, dev_state0,
-                     dev_state1, dev_out_u16, dev_out_f32, (hipStream_t)stream);
+                     dev_state1, dev_out_u16, dev_out_f32, (hipStream_t)stream, fused);
 }
+
+}  // namespace mxb
+
+extern "C" {
 
 int mxb_device_diffusion_noalloc(const uint8_t *dev_mask, int64_t nz, int64_t ny, int64_t nx,
                                  const float *kernel, int64_t kernel_size, int64_t repetitions,

@@ -104,6 +104,62 @@ hipError_t launch_init_lowp(const uint8_t *mask, uint16_t *u, int64_t n, bool bf
 hipError_t launch_lowp_to_f32(const uint16_t *u, float *f, int64_t n, bool bf, hipStream_t s);
 constexpr int kMaxRadiusLowp = 64;
 
+// Extra operands of the split (FP32-class) form.  Its state travels as two
+// FP16 planes, u = hi + lo with hi = f16(u) and lo = f16(u - hi): the low
+// plane sits `in_lo` / `out_lo` elements after the high plane's start.  Each
+// chunk runs three MFMAs, hi * k_hi, hi * k_lo and lo * k_hi (the dropped
+// lo * k_lo is below 2^-22 of the product), on taps pre-scaled by a power of
+// two so that their low parts stay normal FP16 numbers; the accumulator times
+// `scale` (the inverse power of two, exact) is the FP32 value.  fin_f32 /
+// fin_u16: the final pass writes FP32 and/or uint16 outputs at the pass's
+// output strides instead of the two planes.
+struct SplitIO {
+    int64_t in_lo = 0, out_lo = 0;
+    float scale = 1.0f;
+    float *fin_f32 = nullptr;
+    uint16_t *fin_u16 = nullptr;
+};
+
+// Split form (mxb_kernels_lowp_lds.hip, mxb_lowp_dma_body.h).  A pass reads
+// the FP16 hi + lo planes at `in` (lo io.in_lo elements further) and writes
+// planes at `out` (lo io.out_lo further), or, when io.fin_f32 / io.fin_u16
+// are set, the final outputs; `mask` non-null illuminates.  io.scale is set
+// by the launcher.
+bool split_pass_supported(const uint16_t *in, const uint16_t *out, const uint8_t *mask, const MaskBits &bits,
+                          const LineGeom &g, int R, const SplitIO &io);
+hipError_t launch_line_pass_split(const uint16_t *in, uint16_t *out, const uint8_t *mask, const MaskBits &bits,
+                                  const LineGeom &g, const float *taps, int R, const SplitIO &io,
+                                  hipStream_t s);
+// Fused x -> y pass of the 16-bit forms (mxb_fused.hip): the z-pass output
+// `in` (one 16-bit plane, or the split form's two) -> `out` with
+// illumination from the mask bits (and, for the split form with io.fin_*
+// set, the final outputs instead of planes).  precision: MXB_BF16, MXB_FP16 or
+// MXB_FP32_SPLIT.
+bool xy_fused_supported(const uint16_t *in, const uint16_t *out, const MaskBits &bits, int64_t nz, int64_t ny,
+                        int64_t nx, int R, int precision, const SplitIO &io);
+hipError_t launch_xy_fused(const uint16_t *in, uint16_t *out, const MaskBits &bits, int64_t nz, int64_t ny,
+                           int64_t nx, const float *taps, int R, int precision, const SplitIO &io, hipStream_t s);
+// MXB_XY_FUSED=0 runs the three-pass form instead (read at every call; the
+// pass-kernel selection MXB_PASS_KERNEL does too).
+inline bool xy_fusion_enabled() {
+    const char *v = std::getenv("MXB_XY_FUSED");
+    if (v != nullptr && std::strcmp(v, "0") == 0) return false;
+    return !force_ring_kernels() && !force_vgpr_staging();
+}
+
+// Split-form state initialisation: planes from the mask (hi = 1 or 0, lo = 0)
+// or from an FP32 state (hi = f16(u), lo = f16(u - hi)); n voxels, the low
+// plane lo_off elements after the high one.
+hipError_t launch_init_split(const uint8_t *mask, uint16_t *planes, int64_t lo_off, int64_t n, hipStream_t s);
+hipError_t launch_split_state(const float *u, uint16_t *planes, int64_t lo_off, int64_t n, hipStream_t s);
+
+// mxb_device_diffusion with the pass structure chosen: fused = false keeps
+// the three-pass form of the 16-bit and split precisions (the structure of
+// the batched persistent passes, so per-volume runs compare bit for bit).
+int device_diffusion(const uint8_t *dev_mask, int64_t nz, int64_t ny, int64_t nx, const float *kernel,
+                     int64_t kernel_size, int64_t repetitions, int precision, void *dev_state0,
+                     void *dev_state1, uint16_t *dev_out_u16, float *dev_out_f32, void *stream, bool fused);
+
 // Plan the three passes of a step over a (nz, ny, nx) volume.
 // z: A -> A, y: A -> T (T is (z, x, y) order), x: T -> A.
 LineGeom geom_z(int64_t nz, int64_t ny, int64_t nx);

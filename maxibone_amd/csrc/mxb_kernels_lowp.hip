@@ -165,6 +165,28 @@ __global__ __launch_bounds__(256) void k_lowp_to_f32(const uint16_t *__restrict_
         f[i] = Lowp<BF>::to_f32(u[i]);
 }
 
+// Split-form planes from the mask: hi = f16(1) on the mask, else 0; lo = 0.
+__global__ __launch_bounds__(256) void k_init_split(const uint8_t *__restrict__ m, uint16_t *__restrict__ p,
+                                                    int64_t lo_off, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        p[i] = m[i] > 0 ? (uint16_t)0x3c00 : (uint16_t)0;
+        p[lo_off + i] = 0;
+    }
+}
+
+// Split-form planes from an FP32 state.
+__global__ __launch_bounds__(256) void k_split_state(const float *__restrict__ u, uint16_t *__restrict__ p,
+                                                     int64_t lo_off, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint16_t hi, lo;
+        split_f16(u[i], hi, lo);
+        p[i] = hi;
+        p[lo_off + i] = lo;
+    }
+}
+
 // ---------------------------------------------------------------- host side
 static void plan_segments_lowp(LineGeom &g) {
     const int64_t threads = ((g.n_lane + 63) / 64) * 16 * g.n_outer;  // one wave per 16 lines
@@ -242,6 +264,22 @@ hipError_t launch_lowp_to_f32(const uint16_t *u, float *f, int64_t n, bool bf, h
         hipLaunchKernelGGL(k_lowp_to_f32<true>, dim3((unsigned)blocks_for(n)), dim3(256), 0, s, u, f, n);
     else
         hipLaunchKernelGGL(k_lowp_to_f32<false>, dim3((unsigned)blocks_for(n)), dim3(256), 0, s, u, f, n);
+    return hipGetLastError();
+}
+
+}  // namespace mxb
+
+namespace mxb {
+
+hipError_t launch_init_split(const uint8_t *mask, uint16_t *planes, int64_t lo_off, int64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_init_split, dim3((unsigned)blocks_for(n)), dim3(256), 0, s, mask, planes, lo_off, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_state(const float *u, uint16_t *planes, int64_t lo_off, int64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_split_state, dim3((unsigned)blocks_for(n)), dim3(256), 0, s, u, planes, lo_off, n);
     return hipGetLastError();
 }
 

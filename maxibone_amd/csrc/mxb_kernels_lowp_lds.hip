@@ -344,6 +344,123 @@ static hipError_t launch_lowp_staged_rt(const uint16_t *in, uint16_t *out, const
     return launch_lowp_lds_rt<RT>(in, out, mask, g, k, R, bf, s);
 }
 
+// ---------------------------------------------------------------- split form
+// FP32-class state as FP16 hi + lo planes on the direct-load body (SplitIO in
+// mxb_lowp_dma_body.h).  Prefetch depths (tile pairs ahead): the two rings
+// double the LDS of a slot, so the depths are lower than the 16-bit forms';
+// the illuminating pass needs D - 1 to be a multiple of 4.
+#ifndef MXB_SPLIT_DEPTH_Z
+#define MXB_SPLIT_DEPTH_Z 3
+#endif
+#ifndef MXB_SPLIT_DEPTH_Y
+#define MXB_SPLIT_DEPTH_Y 3
+#endif
+#ifndef MXB_SPLIT_DEPTH_X
+#define MXB_SPLIT_DEPTH_X 5
+#endif
+
+template <int RT, int D, bool MASK, bool ZSTAGE, int FIN>
+__global__ __launch_bounds__(256) void k_line_split_dma(const uint16_t *__restrict__ in,
+                                                        uint16_t *__restrict__ out,
+                                                        const uint8_t *__restrict__ mask,
+                                                        MaskBits bits, LineGeom g,
+                                                        TapsL<2 * RT + 1> taps, SplitIO io) {
+    __shared__ __attribute__((aligned(16)))
+    unsigned char smem[LowpDmaSmem<RT, D, MASK, ZSTAGE, true, FIN>::BYTES];
+    int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t seg_id = bid % g.nseg;
+    bid /= g.nseg;
+    const int64_t nlb = (g.n_lane + 63) >> 6;
+    lowp_dma_body<RT, D, false, MASK, ZSTAGE, true, FIN>(in, out, mask, bits, g, taps.hi, taps.lo, bid / nlb,
+                                                         bid % nlb, seg_id, smem, io);
+}
+
+// Which split-form kernel serves a pass: 0 none; 1 line-contiguous (no
+// mask, planes out); 2 position-contiguous planes out; 3 the same,
+// illuminating from mask bits; 4 + fin (fin = 1 FP32, 2 uint16, 3 both):
+// position-contiguous final outputs.
+static int split_form(const uint16_t *in, const uint16_t *out, const uint8_t *mask, const MaskBits &bits,
+                      const LineGeom &g, const SplitIO &io) {
+    if (g.n_lane % 8 != 0 || g.in_outer % 8 != 0 || g.in_ax % 8 != 0 || io.in_lo % 8 != 0 ||
+        reinterpret_cast<uintptr_t>(in) % 16 != 0)
+        return 0;
+    if (mask != nullptr && bits.w == nullptr) return 0;
+    const int fin = (io.fin_f32 != nullptr ? 1 : 0) + (io.fin_u16 != nullptr ? 2 : 0);
+    if (fin != 0) {
+        if (g.out_ax != 1 || g.out_lane % 8 != 0 || g.out_outer % 8 != 0) return 0;
+        if (io.fin_f32 && reinterpret_cast<uintptr_t>(io.fin_f32) % 16 != 0) return 0;
+        if (io.fin_u16 && reinterpret_cast<uintptr_t>(io.fin_u16) % 16 != 0) return 0;
+        return 4 + fin;
+    }
+    if (reinterpret_cast<uintptr_t>(out) % 16 != 0 || io.out_lo % 8 != 0) return 0;
+    if (mask == nullptr && g.out_lane == 1 && g.out_outer % 8 == 0 && g.out_ax % 8 == 0) return 1;
+    if (g.out_ax == 1 && g.out_lane % 8 == 0 && g.out_outer % 8 == 0) return mask == nullptr ? 2 : 3;
+    return 0;
+}
+
+template <int RT>
+static hipError_t launch_split_rt(int form, const uint16_t *in, uint16_t *out, const uint8_t *mask,
+                                  const MaskBits &bits, LineGeom g, const float *k, int R, SplitIO io,
+                                  hipStream_t s) {
+    const int64_t groups = ((g.n_lane + 63) / 64) * g.n_outer;
+    const int64_t target = 256LL * 6;
+    int64_t nseg = groups > 0 ? (target + groups - 1) / groups : 1;
+    const int64_t max_seg = (g.n_ax + 63) / 64;
+    if (nseg > max_seg) nseg = max_seg;
+    if (nseg < 1) nseg = 1;
+    int64_t seg = (g.n_ax + nseg - 1) / nseg;
+    seg = ((seg + 31) / 32) * 32;
+    g.seg = seg;
+    g.nseg = (g.n_ax + seg - 1) / seg;
+    const int64_t blocks = g.nseg * groups;
+    if (blocks == 0) return hipSuccess;
+    float scale = 1.0f;
+    const TapsL<2 * RT + 1> t = split_taps_scaled<RT>(k, R, &scale);
+    io.scale = scale;
+    const dim3 grid((unsigned)blocks), block(256);
+    constexpr int DZ = MXB_SPLIT_DEPTH_Z, DY = MXB_SPLIT_DEPTH_Y, DX = MXB_SPLIT_DEPTH_X;
+    switch (form) {
+    case 1: hipLaunchKernelGGL((k_line_split_dma<RT, DZ, false, true, 0>), grid, block, 0, s, in, out, mask, bits, g, t, io); break;
+    case 2: hipLaunchKernelGGL((k_line_split_dma<RT, DY, false, false, 0>), grid, block, 0, s, in, out, mask, bits, g, t, io); break;
+    case 3: hipLaunchKernelGGL((k_line_split_dma<RT, DX, true, false, 0>), grid, block, 0, s, in, out, mask, bits, g, t, io); break;
+    case 5: hipLaunchKernelGGL((k_line_split_dma<RT, DX, true, false, 1>), grid, block, 0, s, in, out, mask, bits, g, t, io); break;
+    case 6: hipLaunchKernelGGL((k_line_split_dma<RT, DX, true, false, 2>), grid, block, 0, s, in, out, mask, bits, g, t, io); break;
+    case 7: hipLaunchKernelGGL((k_line_split_dma<RT, DX, true, false, 3>), grid, block, 0, s, in, out, mask, bits, g, t, io); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+bool split_pass_supported(const uint16_t *in, const uint16_t *out, const uint8_t *mask, const MaskBits &bits,
+                          const LineGeom &g, int R, const SplitIO &io) {
+    // the final-output forms always illuminate (they end a repetition)
+    if (io.fin_f32 != nullptr || io.fin_u16 != nullptr) {
+        if (mask == nullptr) return false;
+    }
+    return R <= kMaxRadiusLowp && split_form(in, out, mask, bits, g, io) != 0;
+}
+
+hipError_t launch_line_pass_split(const uint16_t *in, uint16_t *out, const uint8_t *mask, const MaskBits &bits,
+                                  const LineGeom &g, const float *taps, int R, const SplitIO &io,
+                                  hipStream_t s) {
+    if (!split_pass_supported(in, out, mask, bits, g, R, io)) return hipErrorInvalidValue;
+    const int form = split_form(in, out, mask, bits, g, io);
+#define MXB_SPLIT_CASE(rt) \
+    if (R <= rt) return launch_split_rt<rt>(form, in, out, mask, bits, g, taps, R, io, s);
+    // the 16-bit forms' radius buckets
+    MXB_SPLIT_CASE(4)
+    MXB_SPLIT_CASE(8)
+    MXB_SPLIT_CASE(12)
+    MXB_SPLIT_CASE(16)
+    MXB_SPLIT_CASE(24)
+    MXB_SPLIT_CASE(32)
+    MXB_SPLIT_CASE(40)
+    MXB_SPLIT_CASE(48)
+    MXB_SPLIT_CASE(64)
+#undef MXB_SPLIT_CASE
+    return hipErrorInvalidValue;
+}
+
 bool lowp_lds_supported(const uint16_t *in, const LineGeom &g, int R) {
     return R <= 64 && g.in_outer % 8 == 0 && g.in_ax % 8 == 0 &&
            reinterpret_cast<uintptr_t>(in) % 16 == 0;

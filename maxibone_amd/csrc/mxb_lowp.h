@@ -17,6 +17,7 @@
 //   B (inputs): X_c[kk = 8g .. 8g+7][j = l%16]
 //   D:          out_j(tile base + 4g + r), r = 0..3
 #pragma once
+#include <cmath>
 #include <cstring>
 
 #include "mxb_internal.h"
@@ -59,6 +60,22 @@ struct Lowp<false> {
         return (float)__builtin_bit_cast(_Float16, v);
     }
 };
+
+// floor(v * 65535.0f) as uint16, through int32 (cpu:191-196).
+__device__ __forceinline__ uint16_t quantise_u16_s(float v) {
+#pragma clang fp contract(off)
+    const float s = v * 65535.0f;
+    return (uint16_t)(int32_t)floorf(s);
+}
+
+// FP16 high and low parts of v for the split (FP32-class) form:
+// hi = f16(v), lo = f16(v - hi), where v - hi is exact in FP32.
+__device__ __forceinline__ void split_f16(float v, uint16_t &hi, uint16_t &lo) {
+#pragma clang fp contract(off)
+    const _Float16 h = (_Float16)v;
+    hi = __builtin_bit_cast(uint16_t, h);
+    lo = __builtin_bit_cast(uint16_t, (_Float16)(v - (float)h));
+}
 
 __device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) {
     return (uint32_t)lo | ((uint32_t)hi << 16);
@@ -146,6 +163,22 @@ TapsL<2 * RT + 1> split_taps(const float *k, int R, bool bf) {
         t.lo[i] = bf ? bf16_bits(rest) : f16_bits(rest);
     }
     return t;
+}
+
+// Taps of the split form: k * 2^shift (shift chosen so that the largest |tap|
+// lies in [2^14, 2^15) and the low parts stay normal FP16 numbers), split into
+// FP16 hi + lo; *scale = 2^-shift undoes the scaling of an accumulator exactly.
+template <int RT>
+TapsL<2 * RT + 1> split_taps_scaled(const float *k, int R, float *scale) {
+    float amax = 0.0f;
+    for (int i = 0; i < 2 * R + 1; ++i) amax = std::fmax(amax, std::fabs(k[i]));
+    int e = 0;
+    if (amax > 0.0f) (void)std::frexp(amax, &e);  // amax in [2^(e-1), 2^e)
+    const int shift = amax > 0.0f ? 15 - e : 0;
+    float kk[2 * 64 + 1];
+    for (int i = 0; i < 2 * R + 1; ++i) kk[i] = std::ldexp(k[i], shift);
+    *scale = std::ldexp(1.0f, -shift);
+    return split_taps<RT>(kk, R, false);
 }
 
 // LDS-staged form (mxb_kernels_lowp_lds.hip): usable when every input row is

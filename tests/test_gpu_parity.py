@@ -3,8 +3,9 @@
 Bars (FP32 state):
 * precision "fp32_exact": bit-identical to the oracle (the reference CPU
   arithmetic: r-ascending sums, product rounded then added).
-* precision "fp32" (MFMA) and "fp32_fma": max |u - u_oracle| <= 1e-5 and
-  uint16 within 1 LSB (src/test/test_diffusion.py:92-98).
+* precision "fp32" (MFMA), "fp32_fma" and "fp32_split" (FP16 hi + lo state,
+  three FP16 MFMAs per chunk): max |u - u_oracle| <= 1e-5 and uint16 within
+  1 LSB (src/test/test_diffusion.py:92-98).
 * drop-in `diffusion` (uint16): within 1 LSB of the oracle and of the scipy
   known answers.
 """
@@ -45,7 +46,7 @@ def _check_f32(gpu, m, k, reps, prec):
     return u, q
 
 
-@pytest.mark.parametrize("prec", ["fp32_exact", "fp32", "fp32_fma"])
+@pytest.mark.parametrize("prec", ["fp32_exact", "fp32", "fp32_fma", "fp32_split"])
 def test_small_fixtures(gpu, golden_small, golden_correlate, prec):
     for name, m, k, reps, u_sc, q_sc in golden_small + golden_correlate:
         _say("case", name, m.shape, len(k), reps, prec)
@@ -79,7 +80,7 @@ def test_radius_sweep(gpu, R):
     m = (rng.random(shape) > 0.995).astype(np.uint8)
     k = gauss_kernel(R / 4.0).astype(np.float32)
     assert len(k) == 2 * R + 1
-    for prec in ("fp32_exact", "fp32", "fp32_fma"):
+    for prec in ("fp32_exact", "fp32", "fp32_fma", "fp32_split"):
         _say("R", R, shape, prec)
         _check_f32(gpu, m, k, 2, prec)
 
@@ -94,7 +95,7 @@ def test_edge_shapes(gpu, shape):
     m = (rng.random(shape) > 0.9).astype(np.uint8)
     for sigma in (1.0, 3.0, 6.0):
         k = gauss_kernel(sigma).astype(np.float32)
-        for prec in ("fp32_exact", "fp32"):
+        for prec in ("fp32_exact", "fp32", "fp32_split"):
             _say("shape", shape, sigma, prec)
             _check_f32(gpu, m, k, 3, prec)
 
@@ -105,7 +106,7 @@ def test_asymmetric_taps(gpu):
     for K in (3, 11, 25, 81):
         k = rng.random(K).astype(np.float32)
         k /= k.sum() * 1.01
-        for prec in ("fp32_exact", "fp32", "fp32_fma"):
+        for prec in ("fp32_exact", "fp32", "fp32_fma", "fp32_split"):
             _say("asym", K, prec)
             _check_f32(gpu, m, k, 3, prec)
 
@@ -137,7 +138,7 @@ def test_chunked_equals_in_memory(gpu, chunk_z, temporal):
     rng = np.random.default_rng(chunk_z * 10 + temporal)
     m = (rng.random((61, 33, 70)) > 0.97).astype(np.uint8)
     k = gauss_kernel(2.2).astype(np.float32)
-    for prec in ("fp32_exact", "fp32"):
+    for prec in ("fp32_exact", "fp32", "fp32_split"):
         _say("chunked", chunk_z, temporal, prec)
         q0, u0 = gpu.diffusion(m, k, 5, precision=prec, f32=True)
         q1, u1 = gpu.diffusion_chunked(m, k, 5, chunk_z, temporal, precision=prec, f32=True)
@@ -219,7 +220,7 @@ def _batch_inputs(seed, n, lo, hi, s_lo, s_hi):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
-def test_batched_host_entry(gpu, prec):
+def test_batched_host_entry(gpu, three_pass, prec):
     masks, kernels = _batch_inputs(31, 6, 5, 70, 0.7, 10.0)
     _say("batched host", prec)
     qs, fs = gpu.diffusion_batched(masks, kernels, 3, precision=prec, f32=True)
@@ -231,7 +232,8 @@ def test_batched_host_entry(gpu, prec):
             assert np.array_equal(f, single)
             assert np.abs(f - u_ref).max() <= F32_TOL and _u16_close(q, q_ref)
         else:
-            # the persistent 16-bit passes equal the single-volume 16-bit path
+            # the persistent 16-bit passes equal the single-volume 16-bit
+            # path in its three-pass form
             q1, f1 = gpu.diffusion(m, k, 3, precision=prec, u16=True, f32=True)
             assert np.array_equal(f, f1) and np.array_equal(q, q1)
             u64 = oracle.diffusion_f64(m, k.astype(np.float64), 3)
@@ -366,6 +368,20 @@ def test_mfma_chain_equals_fma_chain(gpu):
 
 
 @pytest.fixture
+def three_pass():
+    """MXB_XY_FUSED=0 (read by the library at every call): the 16-bit and
+    split forms run their three-pass structure, the one the batched
+    persistent passes share; restores the default afterwards."""
+    old = os.environ.get("MXB_XY_FUSED")
+    os.environ["MXB_XY_FUSED"] = "0"
+    yield
+    if old is None:
+        os.environ.pop("MXB_XY_FUSED", None)
+    else:
+        os.environ["MXB_XY_FUSED"] = old
+
+
+@pytest.fixture
 def pass_kernel():
     """Select the line-pass kernel form through MXB_PASS_KERNEL (read by the
     library at every launch); restores the default afterwards."""
@@ -382,7 +398,7 @@ def pass_kernel():
 
 
 @pytest.mark.parametrize("prec", ["bf16", "fp16", "fp32"])
-def test_kernel_forms_agree(gpu, pass_kernel, prec):
+def test_kernel_forms_agree(gpu, pass_kernel, three_pass, prec):
     """The register-ring, VGPR-staged LDS and direct-load LDS forms of a pass
     group taps into MFMA chunks identically, so they agree bit for bit.
     Shapes cover segment tails (axes not a multiple of 32), partial 64-line
@@ -409,7 +425,7 @@ def test_kernel_forms_agree(gpu, pass_kernel, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
-def test_kernel_forms_agree_random(gpu, pass_kernel, prec):
+def test_kernel_forms_agree_random(gpu, pass_kernel, three_pass, prec):
     """Seeded random shapes and radii (1..64): the default (direct-load) form
     equals the register-ring form bit for bit, twice in a row."""
     rng = np.random.default_rng(2026 if prec == "fp32" else 2027)
@@ -433,8 +449,9 @@ def test_kernel_forms_agree_random(gpu, pass_kernel, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
-def test_batched_random_shapes_equal_single_volume(gpu, prec):
-    """Batched plans over random shapes and radii equal the single-volume path."""
+def test_batched_random_shapes_equal_single_volume(gpu, three_pass, prec):
+    """Batched plans over random shapes and radii equal the single-volume path
+    (its three-pass form for the 16-bit precisions)."""
     rng = np.random.default_rng(31)
     masks, kernels = [], []
     for _ in range(7):
@@ -581,3 +598,95 @@ def test_on_disk_precisions(gpu, tmp_path, prec):
     got = np.fromfile(fout, np.uint16).reshape(shape)
     want = gpu.diffusion(m, k, 5, precision=prec)[0]
     assert np.array_equal(got, want)
+
+
+# ---------------------------------------------------------------- split form
+def test_split_form_takes_the_split_passes(gpu):
+    """On volumes whose x extent is a multiple of 8 the split form runs its
+    own passes (its result differs from the FP32 MFMA chain in the last
+    bits); elsewhere it runs the FP32 form (bit-equal to it)."""
+    rng = np.random.default_rng(55)
+    k = gauss_kernel(5.0).astype(np.float32)
+    m = (rng.random((40, 48, 64)) > 0.96).astype(np.uint8)
+    a = gpu.diffusion(m, k, 4, precision="fp32_split", u16=False, f32=True)[1]
+    b = gpu.diffusion(m, k, 4, precision="fp32", u16=False, f32=True)[1]
+    assert not np.array_equal(a, b) and np.abs(a - b).max() <= 1e-5
+    m = (rng.random((40, 48, 60)) > 0.96).astype(np.uint8)
+    a = gpu.diffusion(m, k, 4, precision="fp32_split", u16=False, f32=True)[1]
+    b = gpu.diffusion(m, k, 4, precision="fp32", u16=False, f32=True)[1]
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("sigma,reps", [(1.0, 30), (3.0, 40), (10.0, 40), (15.0, 12)])
+def test_split_form_long_runs(gpu, sigma, reps):
+    """Many repetitions on a workload crop: the split form stays within the
+    FP32 bar of the oracle (its state carries 22 significant bits)."""
+    m = WORKLOADS["c1"].mask()[:96, :104, :112].copy()
+    k = gauss_kernel(sigma).astype(np.float32)
+    _say("split long run", sigma, reps)
+    _check_f32(gpu, m, k, reps, "fp32_split")
+
+
+def test_split_form_taps_of_any_scale(gpu):
+    """The taps are scaled by a power of two before the FP16 split: kernels
+    with large, tiny and negative taps keep the FP32 bar."""
+    rng = np.random.default_rng(12)
+    m = (rng.random((24, 32, 48)) > 0.95).astype(np.uint8)
+    for scale in (1e-4, 1.0, 3.0):
+        k = (rng.random(21).astype(np.float32) - 0.2) * np.float32(scale)
+        k /= np.float32(max(1.0, float(np.abs(k).sum())))
+        _say("split taps", scale)
+        u_ref, q_ref = oracle.diffusion_f32(m, k, 3)
+        q, u = gpu.diffusion(m, k, 3, precision="fp32_split", u16=True, f32=True)
+        assert np.abs(u - u_ref).max() <= F32_TOL
+
+
+# ---------------------------------------------------------------- fused x -> y pass
+FUSED_SHAPES = [(20, 70, 128), (9, 33, 136), (12, 100, 256), (7, 64, 1000), (5, 130, 72), (3, 1, 8), (4, 257, 16)]
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp32_split"])
+def test_fused_pass_against_three_pass(gpu, prec):
+    """The fused x -> y pass (x then y through LDS) against the three-pass
+    form (y then x through HBM) and the oracle: partial strips (x extent not a
+    multiple of the 128-column strip), row counts not a multiple of 32, single
+    rows, every radius bucket.  The two orders round differently, so the bars
+    are the form's own: 2e-3 of FP64 for the 16-bit forms, 1e-5 and 1 LSB of
+    the FP32 oracle for the split form."""
+    rng = np.random.default_rng(808)
+    for shape in FUSED_SHAPES:
+        m = (rng.random(shape) > 0.93).astype(np.uint8)
+        for sigma in (0.8, 3.0, 10.0, 12.0, 16.0):
+            k = gauss_kernel(sigma).astype(np.float32)
+            reps = 3
+            _say("fused", prec, shape, sigma)
+            q, u = gpu.diffusion(m, k, reps, precision=prec, u16=True, f32=True)
+            os.environ["MXB_XY_FUSED"] = "0"
+            try:
+                q3, u3 = gpu.diffusion(m, k, reps, precision=prec, u16=True, f32=True)
+            finally:
+                os.environ.pop("MXB_XY_FUSED", None)
+            if prec == "fp32_split":
+                u_ref, q_ref = oracle.diffusion_f32(m, k, reps)
+                assert np.abs(u - u_ref).max() <= F32_TOL and _u16_close(q, q_ref)
+                assert np.abs(u - u3).max() <= F32_TOL
+            else:
+                u64 = oracle.diffusion_f64(m, k.astype(np.float64), reps)
+                assert np.abs(u - u64).max() <= LOWP_TOL
+                assert np.abs(u - u3).max() <= 2 * LOWP_TOL
+            assert np.all(u[m > 0] == 1.0)
+
+
+def test_fused_pass_is_taken(gpu):
+    """The default 16-bit path runs the fused pass where it applies: its
+    rounding differs from the three-pass form's."""
+    rng = np.random.default_rng(3)
+    m = (rng.random((16, 64, 128)) > 0.95).astype(np.uint8)
+    k = gauss_kernel(4.0).astype(np.float32)
+    a = gpu.diffusion(m, k, 4, precision="bf16", u16=False, f32=True)[1]
+    os.environ["MXB_XY_FUSED"] = "0"
+    try:
+        b = gpu.diffusion(m, k, 4, precision="bf16", u16=False, f32=True)[1]
+    finally:
+        os.environ.pop("MXB_XY_FUSED", None)
+    assert not np.array_equal(a, b)

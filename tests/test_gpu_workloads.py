@@ -19,7 +19,8 @@ arithmetic in the same order.  Windows sit at corners, on faces, across the
 cylinder surface and, for 2048x2048x1536, at flat offsets beyond 2^31 and
 2^32 voxels.  Bars (src/test/test_diffusion.py:92-98 for FP32):
   fp32_exact: bit-equal to the oracle;
-  fp32 (MFMA): max-abs <= 1e-5 and uint16 within 1 LSB of the oracle;
+  fp32 (MFMA) and fp32_split (FP16 hi + lo state, three FP16 MFMAs per
+  chunk): max-abs <= 1e-5 and uint16 within 1 LSB of the oracle;
   bf16: max-abs <= 2e-3 of the FP64 oracle.
 
 Full-repetition runs check what exact arithmetic fixes: u = 1 on the mask,
@@ -120,7 +121,7 @@ def _check_windows(dev, mask, k, origins, precisions):
 # the state has converged a repetition may lower it by a few ulps; the 16-bit
 # forms round the state to BF16 between repetitions, which can move it by up
 # to their error bar.
-MONO_SLACK = {"fp32": lambda u: 5e-7 * u, "bf16": lambda u: LOWP_TOL}
+MONO_SLACK = {"fp32": lambda u: 5e-7 * u, "fp32_split": lambda u: 5e-7 * u, "bf16": lambda u: LOWP_TOL}
 
 
 def _check_properties(u_prev, u, mask_slab, prec):
@@ -150,7 +151,7 @@ def c2(gpu, workload_mask):
 
 def test_c2_windows(c2):
     w, mask, k, dev = c2
-    _check_windows(dev, mask, k, C2_ORIGINS, ("fp32_exact", "fp32", "bf16"))
+    _check_windows(dev, mask, k, C2_ORIGINS, ("fp32_exact", "fp32", "fp32_split", "bf16"))
 
 
 def test_c2_full_repetitions_properties(c2):
@@ -158,7 +159,7 @@ def test_c2_full_repetitions_properties(c2):
     (repetitions 39 and 40 compared slab by slab)."""
     w, mask, k, dev = c2
     nz = mask.shape[0]
-    for prec in ("fp32", "bf16"):
+    for prec in ("fp32", "fp32_split", "bf16"):
         _say("1024^3 full repetitions", prec)
         dev.run(w.reps - 1, prec)
         prev = dev.df.to_array(np.float32, mask.shape)
@@ -194,12 +195,12 @@ def c4(gpu, workload_mask):
 @pytest.mark.timeout(600)
 def test_c4_windows(c4):
     w, mask, k, dev = c4
-    _check_windows(dev, mask, k, C4_ORIGINS, ("fp32_exact", "fp32"))
+    _check_windows(dev, mask, k, C4_ORIGINS, ("fp32_exact", "fp32", "fp32_split"))
     _check_windows(dev, mask, k, C4_ORIGINS[:5], ("bf16",))
 
 
 @pytest.mark.timeout(600)
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("prec", ["fp32", "fp32_split", "bf16"])
 def test_c4_chunked_equals_in_core(c4, gpu, prec):
     """z-chunks of 256 layers with halo R (one repetition per chunk visit,
     state in host memory between visits): bit-equal to the in-HBM result."""
@@ -223,7 +224,7 @@ def test_c4_full_repetitions_properties(c4):
     state of a slab per 256 layers."""
     w, mask, k, dev = c4
     nz = mask.shape[0]
-    for prec in ("fp32", "bf16"):
+    for prec in ("fp32", "fp32_split", "bf16"):
         _say("2048x2048x1536 full repetitions", prec)
         dev.run(w.reps - 1, prec)
         prev_q = dev.dq.to_array(np.uint16, mask.shape)
@@ -237,7 +238,7 @@ def test_c4_full_repetitions_properties(c4):
             q = dev.dq.box(np.uint16, mask.shape, (z0, 0, 0), (128,) + mask.shape[1:])
             on = mask[z0:z0 + 128] > 0
             assert np.all(q[on] == 65535)
-            lsb = 1 if prec == "fp32" else int(LOWP_TOL * 65535) + 1
+            lsb = int(LOWP_TOL * 65535) + 1 if prec == "bf16" else 1
             assert np.all(q.astype(np.int32) >= prev_q[z0:z0 + 128].astype(np.int32) - lsb)
         del prev_q
 
