@@ -208,8 +208,15 @@ int mxb_device_diffusion_noalloc(const uint8_t *dev_mask, int64_t nz, int64_t ny
 
 /* ---- per-kernel timing (HIP events on the launch stream) ------------------ */
 
-/* Kernel classes reported by mxb_kernel_times. */
-typedef enum { MXB_K_ZPASS = 0, MXB_K_YPASS = 1, MXB_K_XPASS = 2, MXB_K_OTHER = 3 } mxb_kernel_class;
+/* Kernel classes reported by mxb_kernel_times (MXB_K_XYPASS: the fused
+ * x -> y pass of the 16-bit and split forms). */
+typedef enum {
+    MXB_K_ZPASS = 0,
+    MXB_K_YPASS = 1,
+    MXB_K_XPASS = 2,
+    MXB_K_OTHER = 3,
+    MXB_K_XYPASS = 4
+} mxb_kernel_class;
 
 /* Enable (1) / disable (0) event timing of every pass launch; resets totals. */
 int mxb_set_kernel_timing(int enable);

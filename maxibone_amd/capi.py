@@ -17,7 +17,7 @@ HEADERS = [os.path.join(os.path.dirname(PKG), "include", h)
 
 OK, EINVAL, EDEVICE, ENOMEM, EIO = 0, 1, 2, 3, 4
 PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "fp32_fma": 3, "fp32_exact": 4, "fp32_split": 5}
-K_ZPASS, K_YPASS, K_XPASS, K_OTHER = 0, 1, 2, 3
+K_ZPASS, K_YPASS, K_XPASS, K_OTHER, K_XYPASS = 0, 1, 2, 3, 4
 NOALLOC_SCRATCH_BYTES = 20 << 20  # MXB_NOALLOC_SCRATCH_BYTES
 
 _i64 = ctypes.c_int64

@@ -43,8 +43,8 @@ struct TimingState {
     };
     std::vector<Rec> recs;
     std::vector<hipEvent_t> pool;
-    double total_ms[4] = {0, 0, 0, 0};
-    int64_t launches[4] = {0, 0, 0, 0};
+    double total_ms[5] = {0, 0, 0, 0, 0};
+    int64_t launches[5] = {0, 0, 0, 0, 0};
 };
 static TimingState g_timing;
 
@@ -220,7 +220,7 @@ static int run_lowp_core(const uint8_t *d_mask, const MaskBits &bits, int64_t nz
         MXB_HIP(timed(MXB_K_ZPASS, s, [&] { return launch_line_pass_lowp(cur, tmp, nullptr, gz, k, R, bf, s); }));
         if (fused) {
             // x then y in one pass, back into cur, illuminating
-            MXB_HIP(timed(MXB_K_XPASS, s, [&] {
+            MXB_HIP(timed(MXB_K_XYPASS, s, [&] {
                 return launch_xy_fused(tmp, cur, bits, nz, ny, nx, k, R, prec, SplitIO{}, s);
             }));
             continue;
@@ -320,7 +320,7 @@ static int run_steps_split(const uint8_t *d_mask, const MaskBits &bits, int64_t 
             if (final_state) *final_state = ex.fin_f32;
         }
         if (form == 2) {
-            MXB_HIP(timed(MXB_K_XPASS, s, [&] {
+            MXB_HIP(timed(MXB_K_XYPASS, s, [&] {
                 return launch_xy_fused(tmp, cur, bits, nz, ny, nx, k, R, MXB_FP32_SPLIT, ex, s);
             }));
             continue;
@@ -1063,7 +1063,7 @@ int mxb_device_count(void) {
 int mxb_set_kernel_timing(int enable) {
     std::lock_guard<std::mutex> lk(g_timing.mu);
     drain_timing_locked();
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 5; ++i) {
         g_timing.total_ms[i] = 0;
         g_timing.launches[i] = 0;
     }
@@ -1072,7 +1072,7 @@ int mxb_set_kernel_timing(int enable) {
 }
 
 int mxb_kernel_times(int kernel_class, double *total_ms, int64_t *launches) {
-    if (kernel_class < 0 || kernel_class > 3) {
+    if (kernel_class < 0 || kernel_class > MXB_K_XYPASS) {
         set_error("unknown kernel class");
         return EINVAL_;
     }
