@@ -73,7 +73,50 @@ def parse():
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain elementwise copy of 1 GiB (read) + 1 GiB (write), "
                         "the reference kernel for scaling HBM byte counters")
+    p.add_argument("--ceilings", action="store_true",
+                   help="measure the achievable ceilings instead (HBM read / write / copy of 4 GiB and "
+                        "FP32 and BF16 MFMA throughput, with the library's probe kernels) as one JSON line")
     return p.parse_args()
+
+
+def ceilings():
+    """The achievable rates the kernels' rooflines are read against: HBM pure
+    read, pure write and copy (plain and non-temporal) of 4 GiB buffers, the
+    torch elementwise copy of the same size, and dense MFMA throughput."""
+    import ctypes
+
+    from maxibone_amd import capi, hip
+    L = capi.lib()
+    nbytes = 4 << 30
+    src, dst = hip.DeviceBuffer(nbytes), hip.DeviceBuffer(nbytes)
+    src.fill_bytes(0)
+    dst.fill_bytes(0)
+    stream = hip.Stream()
+    rec = {"ceilings": "MI355X achievable rates", "buffer_bytes": nbytes, "hbm_TBps": {}, "mfma_TFLOPs": {}}
+    ms = ctypes.c_float()
+    for name, kind, moved in (("read", 0, 1), ("write", 1, 1), ("copy", 2, 2), ("copy_nontemporal", 3, 2)):
+        capi.check(L.mxb_probe_hbm(kind, src.ptr, dst.ptr, nbytes, 10, stream.handle, ctypes.byref(ms)))
+        rec["hbm_TBps"][name] = round(moved * nbytes / (ms.value * 1e-3) / 1e12, 3)
+    flops = ctypes.c_double()
+    for name, kind, iters in (("fp32_16x16x4", 0, 4096), ("bf16_16x16x32", 1, 4096)):
+        capi.check(L.mxb_probe_mfma(kind, iters, dst.ptr, 5, stream.handle, ctypes.byref(ms), ctypes.byref(flops)))
+        rec["mfma_TFLOPs"][name] = round(flops.value / (ms.value * 1e-3) / 1e12, 1)
+    src.free()
+    dst.free()
+    import torch
+    a = torch.zeros(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    rec["hbm_TBps"]["torch_copy"] = round(2 * nbytes * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e12, 3)
+    rec["peaks"] = {"hbm_TBps": 8.0, "fp32_mfma_TFLOPs": 157.3, "bf16_mfma_TFLOPs": 2500.0}
+    print(json.dumps(rec), flush=True)
 
 
 def calibrate_copy():
@@ -459,6 +502,9 @@ def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.ceilings:
+        ceilings()
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "

@@ -22,7 +22,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(PKG, "_build")
 
-HIP_SOURCES = ["mxb_kernels_f32.hip", "mxb_kernels_f32_lds.hip", "mxb_kernels_lowp.hip", "mxb_kernels_lowp_lds.hip", "mxb_fused.hip", "mxb_batched.hip", "mxb_fields.hip", "mxb_edt.hip", "mxb_capi.hip"]
+HIP_SOURCES = ["mxb_kernels_f32.hip", "mxb_kernels_f32_lds.hip", "mxb_kernels_lowp.hip", "mxb_kernels_lowp_lds.hip", "mxb_fused.hip", "mxb_batched.hip", "mxb_fields.hip", "mxb_edt.hip", "mxb_probe.hip", "mxb_capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

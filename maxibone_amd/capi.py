@@ -1,5 +1,5 @@
 """ctypes bindings to the C ABI of libmxb_diffusion (include/maxibone_diffusion.h,
-include/maxibone_fields.h).
+include/maxibone_fields.h, include/maxibone_edt.h, include/maxibone_probe.h).
 
 The library is loaded from the package directory, where ``maxibone_amd.build``
 places it.  There is no fallback: if the library is missing, ``lib()`` raises.
@@ -13,7 +13,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libmxb_diffusion.so")
 HEADERS = [os.path.join(os.path.dirname(PKG), "include", h)
-           for h in ("maxibone_diffusion.h", "maxibone_fields.h", "maxibone_edt.h")]
+           for h in ("maxibone_diffusion.h", "maxibone_fields.h", "maxibone_edt.h", "maxibone_probe.h")]
 
 OK, EINVAL, EDEVICE, ENOMEM, EIO = 0, 1, 2, 3, 4
 PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "fp32_fma": 3, "fp32_exact": 4, "fp32_split": 5}
@@ -87,6 +87,9 @@ def lib():
             "mxb_edt_field": [_vp, _i64, _i64, _i64, _vp, _vp, ctypes.c_int],
             "mxb_device_edt": [_vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp],
             "mxb_device_edt_field": [_vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp],
+            "mxb_probe_hbm": [ctypes.c_int, _vp, _vp, _i64, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_float)],
+            "mxb_probe_mfma": [ctypes.c_int, _i64, _vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_float),
+                               ctypes.POINTER(ctypes.c_double)],
         }
         for name, args in sig.items():
             f = getattr(L, name)

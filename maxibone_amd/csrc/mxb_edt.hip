@@ -115,8 +115,15 @@ __device__ __forceinline__ int switch_pos(int x0, int c0, int x1, int c1, int n)
 
 // Envelope pass along y or z: uint32 -> uint32, or (LAST) uint32 -> float
 // distance x cylinder with the block's min and max folded into P.minmax.
+// Lines per block that the LDS budget admits for a band length L (the host
+// clamps to it): a line needs 4 (pitch + 64 L + kMetaInts) bytes, over 25 KB
+// once L = 64 (n > 2048) and over 13 KB at L = 32.  The kernel's launch bound
+// follows it, so the long-line instances may hold their per-lane arrays
+// (64 values each at L = 64) in registers without spilling.
+constexpr int edt_max_lines(int L) { return L >= 64 ? 2 : (L >= 32 ? 4 : kMaxLines); }
+
 template <int L, bool LAST>
-__global__ __launch_bounds__(kMaxLines * 64) void k_edt_pass(EdtPass P) {
+__global__ __launch_bounds__(edt_max_lines(L) * 64) void k_edt_pass(EdtPass P) {
     extern __shared__ __attribute__((aligned(16))) uint32_t smem[];
     const int kThreads = P.lines * 64;  // one wave per line
     const int n = (int)P.n;
@@ -304,13 +311,12 @@ __global__ __launch_bounds__(kMaxLines * 64) void k_edt_pass(EdtPass P) {
                 s_cur = s_nxt;
             }
             wave_sync();
+            // the lane's own maximum first; the running maximum is read again
+            // below rather than held, so only res[] stays live (L = 64 would
+            // otherwise need 2 x 64 VGPRs and spill)
             int run = 0;
-            int mv[L];
 #pragma unroll
-            for (int t = 0; t < L; ++t) {
-                run = max(run, (int)stk[t * 64 + lane]);
-                mv[t] = run;
-            }
+            for (int t = 0; t < L; ++t) run = max(run, (int)stk[t * 64 + lane]);
             int carry = run;  // exclusive prefix max over lanes
 #pragma unroll
             for (int d = 1; d < 64; d <<= 1) {
@@ -319,10 +325,12 @@ __global__ __launch_bounds__(kMaxLines * 64) void k_edt_pass(EdtPass P) {
             }
             carry = __shfl_up(carry, 1, 64);
             if (lane == 0) carry = 0;
+            run = carry;
 #pragma unroll
             for (int t = 0; t < L; ++t) {
                 const int i = b0 + t;
-                const int xm = env[max(carry, mv[t])];
+                run = max(run, (int)stk[t * 64 + lane]);
+                const int xm = env[run];
                 res[t] = g[tix(xm)] + (uint32_t)((i - xm) * (i - xm));
             }
         }
@@ -509,6 +517,7 @@ size_t pass_smem(const EdtPass &P) {
 
 template <int L, bool LAST>
 hipError_t launch_l(const EdtPass &P, hipStream_t s) {
+    if (P.lines > edt_max_lines(L)) return hipErrorInvalidValue;  // beyond the launch bound
     const size_t smem = pass_smem(P);
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_edt_pass<L, LAST>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
@@ -544,7 +553,8 @@ EdtPass pass_geom(int a, int64_t nz, int64_t ny, int64_t nx) {
         P.n_inner = ny * nx; P.inner_stride = 1; P.n_outer = 1; P.outer_stride = 0;
     }
     P.pitch = (int)(P.n + P.n / 16 + 1);
-    int lines = kMaxLines;  // one wave per line; the largest count within the block budget
+    // one wave per line; the largest count within the block budget
+    int lines = edt_max_lines((int)band_of(P.n));
     while (lines > 1 && (int64_t)4 * lines * (P.pitch + 64 * band_of(P.n) + kMetaInts) > kBlockBudget)
         lines >>= 1;
     P.lines = lines;

@@ -19,8 +19,10 @@ def gauss_kernel(sigma):
     if not sigma > 0:
         raise ValueError(f"sigma must be positive, got {sigma}")
     radius = round(4.0 * sigma)
-    x = np.arange(-radius, radius + 1, dtype=np.float64)
-    phi = np.exp((-0.5 / (sigma * sigma)) * x * x)
+    # hlp:402-404's order, so the FP64 taps match it bit for bit: the squared
+    # offsets stay integers and are scaled once by the FP64 factor -0.5/sigma^2
+    offsets = np.arange(-radius, radius + 1)
+    phi = np.exp((-0.5 / (sigma * sigma)) * (offsets * offsets))
     return phi / phi.sum()
 
 

@@ -33,6 +33,32 @@ def test_gauss_kernel_matches_scipy_impulse_response():
             np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-17)
 
 
+def test_gauss_kernel_is_bit_exact_with_the_oracle():
+    # hlp:402-404 squares integer offsets and scales them by the FP64 factor
+    # -0.5/sigma^2; the drop-in helper follows that order, so its FP64 taps
+    # equal the oracle's bit for bit (not only to FP64 rounding)
+    sigmas = np.concatenate([np.linspace(0.3, 20.0, 1970), [1.0, 2.2, 3.0, 10.0, 15.0]])
+    for s in sigmas:
+        a, b = gauss_kernel(float(s)), oracle.gauss_kernel(float(s))
+        assert a.dtype == b.dtype == np.float64
+        assert a.tobytes() == b.tobytes(), s
+
+
+def test_gauss_kernel_bit_check_sees_operation_order():
+    # the check above discriminates: scaling float offsets twice, the order a
+    # restatement could plausibly take, changes FP64 bits for most sigmas
+    def float_order(sigma):
+        r = round(4.0 * sigma)
+        x = np.arange(-r, r + 1, dtype=np.float64)
+        phi = np.exp((-0.5 / (sigma * sigma)) * x * x)
+        return phi / phi.sum()
+
+    sigmas = np.linspace(0.3, 20.0, 200)
+    differ = sum(float_order(float(s)).tobytes() != oracle.gauss_kernel(float(s)).tobytes()
+                 for s in sigmas)
+    assert differ > 50
+
+
 @pytest.mark.parametrize("sigma,taps", [(3.0, 25), (10.0, 81), (15.0, 121), (1.125, 9), (0.625, 5),
                                         (1.375, 13), (0.375, 5), (0.875, 9)])
 def test_gauss_kernel_radius_is_round_half_even(sigma, taps):
