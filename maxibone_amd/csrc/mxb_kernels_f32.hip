@@ -399,9 +399,19 @@ hipError_t launch_pack_mask_bits(const uint8_t *mask, int64_t nz, int64_t ny, in
 
 LineGeom geom_z(int64_t nz, int64_t ny, int64_t nx) {
     LineGeom g{};
-    g.n_lane = nx; g.n_outer = ny; g.n_ax = nz;
-    g.in_outer = nx; g.in_ax = ny * nx;
-    g.out_lane = 1; g.out_outer = nx; g.out_ax = ny * nx;
+    g.n_ax = nz; g.in_ax = ny * nx; g.out_lane = 1; g.out_ax = ny * nx;
+    if (nx % 64 != 0 && (ny * nx) % 64 == 0) {
+        // rows are not whole 64-lane groups but planes are: the z-pass reads
+        // no mask and every (y, x) lane is an independent line, so a plane is
+        // one run of ny * nx lanes, and each group is a whole aligned piece
+        // instead of every row ending in a partial group whose cache lines
+        // the next group reads and writes again
+        g.n_lane = ny * nx; g.n_outer = 1;
+        g.in_outer = ny * nx; g.out_outer = ny * nx;
+    } else {
+        g.n_lane = nx; g.n_outer = ny;
+        g.in_outer = nx; g.out_outer = nx;
+    }
     return g;
 }
 

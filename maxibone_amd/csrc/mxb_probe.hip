@@ -59,16 +59,19 @@ constexpr int kMfmaChains = 8;
 constexpr int kMfmaBlocks = 1024;  // x 4 waves: four waves per SIMD on 256 CUs
 
 template <int KIND>
-__global__ __launch_bounds__(kProbeThreads) void k_probe_mfma(int64_t iters, float *sink) {
+__global__ __launch_bounds__(kProbeThreads) void k_probe_mfma(int iters, float *sink) {
     const float f = 1.0f + 1e-3f * (float)(threadIdx.x & 63);
     v4f acc[kMfmaChains];
 #pragma unroll
-    for (int c = 0; c < kMfmaChains; ++c) acc[c] = v4f{0.f, 0.f, 0.f, 0.f};
-    if (KIND == MXB_PROBE_MFMA_F32) {
+    for (int c = 0; c < kMfmaChains; ++c) acc[c] = v4f{f + (float)c, 0.f, 0.f, 0.f};
+    // one scheduling barrier per trip keeps each chain in its own registers
+    // (without it the 16-bit form's accumulators were rotated through copies)
+    if constexpr (KIND == MXB_PROBE_MFMA_F32) {
         const float a = f, b = 1.0f - f;
-        for (int64_t i = 0; i < iters; ++i) {
+        for (int i = 0; i < iters; ++i) {
 #pragma unroll
             for (int c = 0; c < kMfmaChains; ++c) acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[c], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
         }
     } else {
         v8bf a, b;
@@ -77,15 +80,16 @@ __global__ __launch_bounds__(kProbeThreads) void k_probe_mfma(int64_t iters, flo
             a[j] = (__bf16)(f + (float)j);
             b[j] = (__bf16)(f - (float)j);
         }
-        for (int64_t i = 0; i < iters; ++i) {
+        for (int i = 0; i < iters; ++i) {
 #pragma unroll
             for (int c = 0; c < kMfmaChains; ++c) acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[c], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
-    float s = 0.f;
+    v4f t = acc[0];
 #pragma unroll
-    for (int c = 0; c < kMfmaChains; ++c) s += acc[c].x + acc[c].y + acc[c].z + acc[c].w;
-    if (s == 1234.5f) sink[0] = s;  // keeps the products; the sums never equal it
+    for (int c = 1; c < kMfmaChains; ++c) t += acc[c];
+    if (t.x + t.y + t.z + t.w == 1234.5f) sink[0] = t.x;  // keeps the products; the sums never equal it
 }
 
 // Times `reps` launches of `launch` after one untimed launch.
@@ -96,18 +100,18 @@ int timed_launches(F launch, int reps, hipStream_t s, float *ms) {
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = MXB_EDEVICE;
     if (rc == MXB_OK) {
         launch();
-        hipEventRecord(e0, s);
+        bool ok = hipEventRecord(e0, s) == hipSuccess;
         for (int r = 0; r < reps; ++r) launch();
-        hipEventRecord(e1, s);
+        ok = ok && hipEventRecord(e1, s) == hipSuccess;
         float t = 0.f;
-        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        if (!ok || hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
             hipEventElapsedTime(&t, e0, e1) != hipSuccess)
             rc = MXB_EDEVICE;
         else
             *ms = t / (float)reps;
     }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
+    if (e0 && hipEventDestroy(e0) != hipSuccess) rc = MXB_EDEVICE;
+    if (e1 && hipEventDestroy(e1) != hipSuccess) rc = MXB_EDEVICE;
     return rc;
 }
 
@@ -142,16 +146,17 @@ extern "C" int mxb_probe_hbm(int kind, const void *src, void *dst, int64_t bytes
 extern "C" int mxb_probe_mfma(int kind, int64_t iters, void *sink, int reps, void *stream, float *ms_per_launch,
                               double *flops_per_launch) {
     using namespace mxb;
-    if (ms_per_launch == nullptr || flops_per_launch == nullptr || sink == nullptr || iters < 1 || reps < 1)
+    if (ms_per_launch == nullptr || flops_per_launch == nullptr || sink == nullptr || iters < 1 ||
+        iters > (1 << 24) || reps < 1)
         return MXB_EINVAL;
     if (kind != MXB_PROBE_MFMA_F32 && kind != MXB_PROBE_MFMA_BF16) return MXB_EINVAL;
     hipStream_t s = static_cast<hipStream_t>(stream);
     float *out = static_cast<float *>(sink);
     auto launch = [&]() {
         if (kind == MXB_PROBE_MFMA_F32)
-            hipLaunchKernelGGL(k_probe_mfma<MXB_PROBE_MFMA_F32>, dim3(kMfmaBlocks), dim3(kProbeThreads), 0, s, iters, out);
+            hipLaunchKernelGGL(k_probe_mfma<MXB_PROBE_MFMA_F32>, dim3(kMfmaBlocks), dim3(kProbeThreads), 0, s, (int)iters, out);
         else
-            hipLaunchKernelGGL(k_probe_mfma<MXB_PROBE_MFMA_BF16>, dim3(kMfmaBlocks), dim3(kProbeThreads), 0, s, iters, out);
+            hipLaunchKernelGGL(k_probe_mfma<MXB_PROBE_MFMA_BF16>, dim3(kMfmaBlocks), dim3(kProbeThreads), 0, s, (int)iters, out);
     };
     // 2 M N K flops per MFMA: 16x16x4 -> 2048, 16x16x32 -> 16384
     const double per = kind == MXB_PROBE_MFMA_F32 ? 2048.0 : 16384.0;

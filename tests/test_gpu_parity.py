@@ -336,6 +336,28 @@ def test_unaligned_rows_direct_load(gpu):
         assert mx <= LOWP_TOL
 
 
+# planes that are whole 64-lane groups with rows that are not: the z-pass runs
+# each plane as one run of ny * nx lanes
+FLAT_Z_SHAPES = [(24, 48, 100), (9, 64, 125), (13, 16, 1000), (6, 192, 40)]
+
+
+@pytest.mark.parametrize("prec", ["fp32_exact", "fp32", "fp32_split", "bf16"])
+def test_flat_plane_z_pass(gpu, prec):
+    """Volumes whose z-pass takes the flat plane layout: the exact form stays
+    bit-identical to the oracle, the others within their bars."""
+    rng = np.random.default_rng(64)
+    for shape in FLAT_Z_SHAPES:
+        m = (rng.random(shape) > 0.94).astype(np.uint8)
+        for sigma in (1.0, 5.0):
+            k = gauss_kernel(sigma).astype(np.float32)
+            _say("flat z", prec, shape, sigma)
+            if prec == "bf16":
+                mx, _, _, _ = _lowp_errors(gpu, m, k, 5, prec)
+                assert mx <= LOWP_TOL
+            else:
+                _check_f32(gpu, m, k, 5, prec)
+
+
 @pytest.mark.parametrize("prec", ["bf16", "fp16"])
 def test_precision_form_one_repetition_is_fp32(gpu, prec):
     m = WORKLOADS["c1"].mask()[:40, :50, :60].copy()
