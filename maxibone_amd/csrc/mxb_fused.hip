@@ -48,6 +48,27 @@ struct XYGeom {
 
 constexpr int kXYWaves = 8;  // waves per block
 
+// Staged rows are XOR-swizzled in 16-byte pieces over groups of this many
+// consecutive rows (a power of two), so that the x-pass's 16-byte operand
+// reads of that many consecutive rows (one per lane) fall in distinct banks.
+#ifndef MXB_XY_SWZ
+#define MXB_XY_SWZ 16
+#endif
+constexpr int kXYSwz = MXB_XY_SWZ - 1;
+
+// Epilogue form.  Wave epilogue: each wave stages and stores its own
+// 16-column x 32-row y-pass tile through a wave-private LDS area, with no
+// block barrier before the stores (rows leave in 32-byte pieces per wave).
+// Block epilogue: the block stages the pair's 32 rows x TX outputs and
+// stores whole rows after a barrier.  1 (measured on 1024^3, R=40): the wave
+// epilogue for the split form (fused pass 2.52 against 2.64 ms), the block
+// epilogue for the 16-bit forms (1.375 against 1.42 ms); 0: block always;
+// 2: wave always.
+#ifndef MXB_XY_WAVE_EPI
+#define MXB_XY_WAVE_EPI 1
+#endif
+static_assert((MXB_XY_SWZ & kXYSwz) == 0 && MXB_XY_SWZ <= 16, "swizzle group: a power of two up to 16");
+
 template <int RT, int TX, int D, bool SPLIT, int FIN>
 struct XYShape {
     static constexpr int NW = kXYWaves;
@@ -72,7 +93,11 @@ struct XYShape {
     static constexpr int RING = W * P * SLOT;
     static constexpr int SPITCH16 = 2 * TX + 16;      // epilogue stage pitch (bytes), 16-bit
     static constexpr int SPITCHF = 4 * TX + 16;       // epilogue stage pitch (bytes), FP32
-    static constexpr int EST = 32 * (FIN != 0 ? SPITCHF : P * SPITCH16);
+    static constexpr int WP16 = 48;                   // wave stage pitch (bytes), 16-bit: 16 columns + pad
+    static constexpr int WPF = 80;                    // wave stage pitch (bytes), FP32
+    static constexpr int WEST = 32 * (FIN != 0 ? WPF : P * WP16);  // wave stage bytes
+    static constexpr bool WEPI = MXB_XY_WAVE_EPI == 2 || (MXB_XY_WAVE_EPI == 1 && SPLIT);
+    static constexpr int EST = WEPI ? NW * WEST : 32 * (FIN != 0 ? SPITCHF : P * SPITCH16);
     static constexpr int MRING = (D + 1) * MSLOT + 256;  // + the target of the idle waves' mask loads
     static constexpr int TAPS = 2 * LowpChunks<RT>::TW * 2;
     static constexpr int BYTES = STAGE + RING + EST + MRING + TAPS;
@@ -83,6 +108,9 @@ struct XYShape {
                                        : P * PIECES16;
     static_assert(TX % (16 * NW) == 0 && (16 * PR) % (64 * NW) == 0 && PIECES16 >= 1,
                   "whole tiles, loads and stores per wave");
+    static_assert(PR % MXB_XY_SWZ == 0, "the swizzle keeps a piece within its row");
+    static_assert(!WEPI || (NQ == 1 && PIECES16 == 1),
+                  "wave epilogue: one 16-column group per wave, one 8-column piece per lane");
     static_assert(RT % 4 == 0, "radius bucket");
     static_assert(FIN == 0 || SPLIT, "final outputs are the split form's");
     static_assert(MLOADS <= 4, "mask words of 16 rows in one load per wave");
@@ -93,6 +121,60 @@ struct XYShape {
     static_assert(D >= 1 && (D - 1) * GL + NST * ST < 64, "vmcnt range");
     static constexpr bool FITS = READS_IN_ROW && BYTES <= 160 * 1024;  // gfx950 LDS per workgroup
 };
+
+// LDS ordering between the lanes of one wave: its LDS operations execute in
+// order; the fences keep the compiler from moving them across.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Final outputs of one 8-column piece: mask voxels (bits of m) become 1, then
+// the FP32 state (FIN & 1) and / or the uint16 result (FIN & 2) are stored;
+// pieces outside the volume go to the sink (the store count stays fixed).
+template <int FIN>
+__device__ __forceinline__ void xy_store_fin(v4f (&v)[2], uint32_t m, bool ok, int64_t off, const SplitIO &io,
+                                             float *sinkf, uint16_t *sink) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        if ((m >> e) & 1u) v[0][e] = 1.0f;
+        if ((m >> (4 + e)) & 1u) v[1][e] = 1.0f;
+    }
+    if constexpr ((FIN & 1) != 0) {
+        float *dst = ok ? io.fin_f32 + off : sinkf;
+        store16<false>(dst, v[0]);
+        store16<false>(dst + (ok ? 4 : 0), v[1]);
+    }
+    if constexpr ((FIN & 2) != 0) {
+        uint32_t wq[4];
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+            wq[2 * hh] = (uint32_t)quantise_u16_s(v[hh][0]) | ((uint32_t)quantise_u16_s(v[hh][1]) << 16);
+            wq[2 * hh + 1] = (uint32_t)quantise_u16_s(v[hh][2]) | ((uint32_t)quantise_u16_s(v[hh][3]) << 16);
+        }
+        store16<false>(ok ? io.fin_u16 + off : sink, make_uint4(wq[0], wq[1], wq[2], wq[3]));
+    }
+}
+
+// 16-bit state of one 8-column piece (v; the split form's low plane vl):
+// halves whose mask bit is set become 1.0 (low plane 0), then stored.
+template <bool BF, bool SPLIT>
+__device__ __forceinline__ void xy_store_state(uint4 v, uint4 vl, uint32_t m, bool ok, int64_t off, uint16_t *out,
+                                               const SplitIO &io, uint16_t *sink) {
+    using L = Lowp<BF>;
+    const uint32_t one2 = L::from_f32(1.0f) * 0x10001u;  // 1.0 in both halves
+    uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    uint32_t wl[4] = {vl.x, vl.y, vl.z, vl.w};
+#pragma unroll
+    for (int e2 = 0; e2 < 4; ++e2) {
+        const uint32_t sel = ((m >> (2 * e2)) & 1u) * 0xffffu | ((m >> (2 * e2 + 1)) & 1u) * 0xffff0000u;
+        w[e2] = (w[e2] & ~sel) | (one2 & sel);
+        wl[e2] &= ~sel;
+    }
+    store16<false>(ok ? out + off : sink, make_uint4(w[0], w[1], w[2], w[3]));
+    if constexpr (SPLIT) store16<false>(ok ? out + io.out_lo + off : sink, make_uint4(wl[0], wl[1], wl[2], wl[3]));
+}
 
 // Steps of 16 rows: step j stages input rows ya - RT + 16 j .. +15 and the
 // x-pass writes them to half j % 2 of ring slot (j / 2) % W; the y-pass of
@@ -140,8 +222,8 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
 
     // ---- step loads: lane slot lq = (n NW + wv) 64 + lane of a plane's
     // staging is row lq / PR, LDS piece lq % PR, which holds the global piece
-    // (lq % PR) ^ (row & 7) of that row (swizzled so that the x-pass operand
-    // reads of 8 consecutive rows fall in distinct bank groups).  The row and
+    // (lq % PR) ^ (row & kXYSwz) of that row (swizzled so that the x-pass
+    // operand reads of consecutive rows fall in distinct bank groups).  The row and
     // column of each of the lane's loads are fixed: only the step's first row
     // moves.
     constexpr int NW = S::NW;
@@ -153,7 +235,7 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
 #pragma unroll
     for (int n = 0; n < NI; ++n) {
         const int lq = (n * NW + wv) * 64 + lane;
-        const int pc = (lq % PR) ^ ((lq / PR) & 7);
+        const int pc = (lq % PR) ^ ((lq / PR) & kXYSwz);
         lrow[n] = lq / PR;
         lcol[n] = x0 - S::HALO + 8 * pc;
         lcol_ok[n] = pc >= S::PLO && pc < PR - S::PLO && lcol[n] >= 0 && lcol[n] < nx;
@@ -256,7 +338,7 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
                 v8s x, xl;
                 if constexpr (RT % 8 == 0) {
                     // one whole 16-byte piece: conflict-free across the wave
-                    const int boff = ((o >> 3) ^ (r & 7)) << 4;
+                    const int boff = ((o >> 3) ^ (r & kXYSwz)) << 4;
                     x = *reinterpret_cast<const v8s *>(rowp + boff);
                     if constexpr (SPLIT) xl = *reinterpret_cast<const v8s *>(rowp + SB + boff);
                 } else {
@@ -264,7 +346,7 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
 #pragma unroll
                     for (int hh = 0; hh < 2; ++hh) {
                         const int oh = o + 4 * hh;
-                        const int boff = (((oh >> 3) ^ (r & 7)) << 4) + ((oh & 7) << 1);
+                        const int boff = (((oh >> 3) ^ (r & kXYSwz)) << 4) + ((oh & 7) << 1);
                         part[hh] = *reinterpret_cast<const uint2 *>(rowp + boff);
                         if constexpr (SPLIT) plo[hh] = *reinterpret_cast<const uint2 *>(rowp + SB + boff);
                     }
@@ -345,6 +427,54 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
                     }
                 }
 
+        const unsigned char *const msl = mring + (p % (D + 1)) * MSLOT;
+        if constexpr (S::WEPI) {
+            // ---- epilogue, per wave: lane (kg, col) holds column
+            // 64 sb + 16 quad + col of the strip, rows 16 u + 4 kg .. +3 of the
+            // pair; the wave stages its 32 x 16 tile as [row][column] in its own
+            // area, then lane l stores row l / 2, columns 8 (l % 2) .. +7 (the
+            // same wave's next pair writes the area only after these reads)
+            unsigned char *const west = est + wv * S::WEST;
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int row = 16 * u + 4 * kg + e;
+                    if constexpr (FIN != 0) {
+                        *reinterpret_cast<float *>(west + row * S::WPF + 4 * col) = acc[0][u][e] * io.scale;
+                    } else if constexpr (SPLIT) {
+                        uint16_t hv, lv;
+                        split_f16(acc[0][u][e] * io.scale, hv, lv);
+                        *reinterpret_cast<uint16_t *>(west + row * S::WP16 + 2 * col) = hv;
+                        *reinterpret_cast<uint16_t *>(west + 32 * S::WP16 + row * S::WP16 + 2 * col) = lv;
+                    } else {
+                        *reinterpret_cast<uint16_t *>(west + row * S::WP16 + 2 * col) = L::from_f32(acc[0][u][e]);
+                    }
+                }
+            wave_lds_sync();
+            const int rw = lane >> 1, hf = lane & 1;
+            const int pc = 8 * (wv >> 2) + 2 * quad + hf;  // 8-column piece of the strip
+            const int64_t y = ya + 32 * (int64_t)p + rw;
+            const int64_t x = x0 + 8 * pc;
+            const bool ok = y < yb && x < nx;
+            const int64_t off = (z * ny + y) * nx + x;
+            const uint32_t m =
+                *reinterpret_cast<const uint32_t *>(msl + (rw >> 4) * MHALF + (pc >> 2) * 64 + (rw & 15) * 4) >>
+                (8 * (pc & 3));
+            if constexpr (FIN != 0) {
+                v4f v[2];
+#pragma unroll
+                for (int hh = 0; hh < 2; ++hh)
+                    v[hh] = *reinterpret_cast<const v4f *>(west + rw * S::WPF + 32 * hf + 16 * hh);
+                xy_store_fin<FIN>(v, m, ok, off, io, sinkf, sink);
+            } else {
+                const uint4 v = *reinterpret_cast<const uint4 *>(west + rw * S::WP16 + 16 * hf);
+                uint4 vl = make_uint4(0, 0, 0, 0);
+                if constexpr (SPLIT) vl = *reinterpret_cast<const uint4 *>(west + 32 * S::WP16 + rw * S::WP16 + 16 * hf);
+                xy_store_state<BF, SPLIT>(v, vl, m, ok, off, out, io, sink);
+            }
+            continue;
+        }
         // ---- epilogue: lane (kg, col) of group gq holds column
         // 64 sb + 16 quad + col, rows 16 u + 4 kg .. +3 of the pair; stage as
         // [row][column] (the previous pair's stores read the stage before
@@ -369,7 +499,6 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
                     }
                 }
         __syncthreads();
-        const unsigned char *const msl = mring + (p % (D + 1)) * MSLOT;
         // 16-bit pieces: piece pc of row rw covers columns 8 pc .. +7
 #pragma unroll
         for (int k = 0; k < S::PIECES16; ++k) {
@@ -387,43 +516,13 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
 #pragma unroll
                 for (int hh = 0; hh < 2; ++hh)
                     v[hh] = *reinterpret_cast<const v4f *>(est + rw * S::SPITCHF + 32 * pc + 16 * hh);
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    if ((m >> e) & 1u) v[0][e] = 1.0f;
-                    if ((m >> (4 + e)) & 1u) v[1][e] = 1.0f;
-                }
-                if constexpr ((FIN & 1) != 0) {
-                    float *dst = ok ? io.fin_f32 + off : sinkf;
-                    store16<false>(dst, v[0]);
-                    store16<false>(dst + (ok ? 4 : 0), v[1]);
-                }
-                if constexpr ((FIN & 2) != 0) {
-                    uint32_t wq[4];
-#pragma unroll
-                    for (int hh = 0; hh < 2; ++hh) {
-                        wq[2 * hh] = (uint32_t)quantise_u16_s(v[hh][0]) | ((uint32_t)quantise_u16_s(v[hh][1]) << 16);
-                        wq[2 * hh + 1] = (uint32_t)quantise_u16_s(v[hh][2]) | ((uint32_t)quantise_u16_s(v[hh][3]) << 16);
-                    }
-                    store16<false>(ok ? io.fin_u16 + off : sink, make_uint4(wq[0], wq[1], wq[2], wq[3]));
-                }
+                xy_store_fin<FIN>(v, m, ok, off, io, sinkf, sink);
             } else {
-                uint4 v = *reinterpret_cast<const uint4 *>(est + rw * S::SPITCH16 + 16 * pc);
+                const uint4 v = *reinterpret_cast<const uint4 *>(est + rw * S::SPITCH16 + 16 * pc);
                 uint4 vl = make_uint4(0, 0, 0, 0);
                 if constexpr (SPLIT)
                     vl = *reinterpret_cast<const uint4 *>(est + 32 * S::SPITCH16 + rw * S::SPITCH16 + 16 * pc);
-                const uint32_t one2 = L::from_f32(1.0f) * 0x10001u;  // 1.0 in both halves
-                uint32_t wv4[4] = {v.x, v.y, v.z, v.w};
-                uint32_t wl[4] = {vl.x, vl.y, vl.z, vl.w};
-#pragma unroll
-                for (int e2 = 0; e2 < 4; ++e2) {
-                    // halves whose mask bit is set become 1.0 (low plane 0)
-                    const uint32_t sel = ((m >> (2 * e2)) & 1u) * 0xffffu | ((m >> (2 * e2 + 1)) & 1u) * 0xffff0000u;
-                    wv4[e2] = (wv4[e2] & ~sel) | (one2 & sel);
-                    wl[e2] &= ~sel;
-                }
-                store16<false>(ok ? out + off : sink, make_uint4(wv4[0], wv4[1], wv4[2], wv4[3]));
-                if constexpr (SPLIT)
-                    store16<false>(ok ? out + io.out_lo + off : sink, make_uint4(wl[0], wl[1], wl[2], wl[3]));
+                xy_store_state<BF, SPLIT>(v, vl, m, ok, off, out, io, sink);
             }
         }
     }
