@@ -114,9 +114,13 @@ struct XYShape {
     static_assert(RT % 4 == 0, "radius bucket");
     static_assert(FIN == 0 || SPLIT, "final outputs are the split form's");
     static_assert(MLOADS <= 4, "mask words of 16 rows in one load per wave");
-    // every chunk read stays in its staged row: the last tile's last chunk
-    // ends at staged half HALO - RT + TX + 32 C - 16
-    static constexpr bool READS_IN_ROW = HALO - RT + TX + 32 * C - 16 <= ROWH && RT <= HALO;
+    // the last tile's last chunk ends at staged half HALO - RT + TX + 32 C - 16;
+    // where that passes the row end (radius bucket 64, whose 32-position chunks
+    // overhang the 16 + 2 RT positions a tile needs by 16), the x-pass reads
+    // zeros for the overhang, which meets only zero taps.  The positions a
+    // tile needs end at HALO + TX + RT, inside the row when RT <= HALO.
+    static constexpr bool GUARD = HALO - RT + TX + 32 * C - 16 > ROWH;
+    static constexpr bool READS_IN_ROW = RT <= HALO;
     static constexpr int NST = D / 2 + 1;  // most epilogues between a group and its wait (every second step)
     static_assert(D >= 1 && (D - 1) * GL + NST * ST < 64, "vmcnt range");
     static constexpr bool FITS = READS_IN_ROW && BYTES <= 160 * 1024;  // gfx950 LDS per workgroup
@@ -341,7 +345,16 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
                     const int boff = ((o >> 3) ^ (r & kXYSwz)) << 4;
                     x = *reinterpret_cast<const v8s *>(rowp + boff);
                     if constexpr (SPLIT) xl = *reinterpret_cast<const v8s *>(rowp + SB + boff);
+                    if constexpr (S::GUARD) {
+                        // past the row end (the read stayed inside the LDS:
+                        // the next row, or the ring after the last buffer)
+                        if (o >= S::ROWH) {
+                            x = v8s{0, 0, 0, 0, 0, 0, 0, 0};
+                            if constexpr (SPLIT) xl = x;
+                        }
+                    }
                 } else {
+                    static_assert(!S::GUARD, "overhang guard for whole 16-byte pieces only");
                     uint2 part[2], plo[2];
 #pragma unroll
                     for (int hh = 0; hh < 2; ++hh) {
@@ -544,12 +557,30 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
 #define MXB_XY_DS 3
 #endif
 
+// Strip width and prefetch depth of a form and radius bucket: the deepest
+// prefetch up to the form's default whose shape fits the LDS (the split form
+// at bucket 48 takes 2, at bucket 64 1).  FITS: the fused pass serves the
+// bucket.  The 16-bit forms take it only where two blocks share a CU (80 KB
+// of LDS each; buckets up to 40): at buckets 48 and 64 one block per CU made
+// it slower than the three-pass form (1024^3 BF16, sigma 12: 251.7 against
+// 260.0 Gvoxel*iter/s; sigma 15: 232.4 against 241.3), while the split form
+// gains there (195.2 against 159.8; 160.6 against 135.4).
+template <int RT, bool SPLIT>
+struct XYForm {
+    static constexpr int TX = SPLIT ? MXB_XY_TXS : MXB_XY_TX16;
+    static constexpr int D0 = SPLIT ? MXB_XY_DS : MXB_XY_D16;
+    static constexpr int D = XYShape<RT, TX, D0, SPLIT, 0>::FITS ? D0
+                             : (D0 > 2 && XYShape<RT, TX, 2, SPLIT, 0>::FITS ? 2 : 1);
+    static constexpr bool FITS = XYShape<RT, TX, D, SPLIT, 0>::FITS &&
+                                 XYShape<RT, TX, D, SPLIT, SPLIT ? 3 : 0>::FITS &&
+                                 (SPLIT || XYShape<RT, TX, D, SPLIT, 0>::BYTES <= 80 * 1024);
+};
+
 // Whether the form's kernel for radius R fits the LDS.
 template <bool SPLIT>
 static bool xy_fits(int R) {
 #define MXB_XY_FITS(rt)                                                                                  \
-    if (R <= rt)                                                                                         \
-        return XYShape<rt, SPLIT ? MXB_XY_TXS : MXB_XY_TX16, SPLIT ? MXB_XY_DS : MXB_XY_D16, SPLIT, 0>::FITS;
+    if (R <= rt) return XYForm<rt, SPLIT>::FITS;
     MXB_XY_FITS(4)
     MXB_XY_FITS(8)
     MXB_XY_FITS(12)
@@ -568,7 +599,6 @@ static bool xy_fits(int R) {
 bool xy_fused_supported(const uint16_t *in, const uint16_t *out, const MaskBits &bits, int64_t nz, int64_t ny,
                         int64_t nx, int R, int precision, const SplitIO &io) {
     if (R > kMaxRadiusLowp || nz * ny * nx == 0 || nx % 8 != 0 || bits.w == nullptr) return false;
-    // (the split form's two planes of staging and ring exceed it at radius bucket 64)
     if (!(precision == MXB_FP32_SPLIT ? xy_fits<true>(R) : xy_fits<false>(R))) return false;
     if (precision != MXB_FP32_SPLIT && (io.fin_f32 != nullptr || io.fin_u16 != nullptr)) return false;
     if (reinterpret_cast<uintptr_t>(in) % 16 != 0 || reinterpret_cast<uintptr_t>(out) % 16 != 0) return false;
@@ -581,8 +611,8 @@ bool xy_fused_supported(const uint16_t *in, const uint16_t *out, const MaskBits 
 template <int RT, bool BF, bool SPLIT>
 static hipError_t launch_xy_rt(const uint16_t *in, uint16_t *out, const MaskBits &bits, int64_t nz, int64_t ny,
                                int64_t nx, const float *k, int R, SplitIO io, hipStream_t s) {
-    constexpr int TX = SPLIT ? MXB_XY_TXS : MXB_XY_TX16;
-    constexpr int D = SPLIT ? MXB_XY_DS : MXB_XY_D16;
+    constexpr int TX = XYForm<RT, SPLIT>::TX;
+    constexpr int D = XYForm<RT, SPLIT>::D;
     XYGeom g;
     g.nz = nz;
     g.ny = ny;
@@ -610,7 +640,7 @@ static hipError_t launch_xy_rt(const uint16_t *in, uint16_t *out, const MaskBits
     const int fin = (io.fin_f32 != nullptr ? 1 : 0) + (io.fin_u16 != nullptr ? 2 : 0);
     const dim3 grid((unsigned)blocks), block(64 * kXYWaves);
     const uint32_t *w = bits.w;
-    if constexpr (!XYShape<RT, TX, D, SPLIT, SPLIT ? 3 : 0>::FITS) {
+    if constexpr (!XYForm<RT, SPLIT>::FITS) {
         return hipErrorInvalidValue;
     } else if constexpr (SPLIT) {
         switch (fin) {
