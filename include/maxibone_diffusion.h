@@ -46,8 +46,8 @@ typedef enum {
                            Same HBM bytes as FP32, a fraction of its MFMA
                            time; within the FP32 bar (1e-5, 1 uint16 LSB) of
                            the reference CPU path.  Volumes whose x extent
-                           is not a multiple of 8 (with radius above 48:
-                           x or y extent) run MXB_FP32.                    */
+                           is not a multiple of 8, and radii above 64, run
+                           MXB_FP32.                                       */
 } mxb_precision;
 
 /* ---- drop-in entry points (host buffers) ------------------------------- */
