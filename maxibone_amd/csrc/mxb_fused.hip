@@ -67,6 +67,13 @@ constexpr int kXYSwz = MXB_XY_SWZ - 1;
 #ifndef MXB_XY_WAVE_EPI
 #define MXB_XY_WAVE_EPI 1
 #endif
+
+// 1: each pass issues all of a wave's operand reads for the step (x-pass) or
+// pair (y-pass) before its MFMAs, so the LDS latencies overlap instead of
+// each read waiting on the previous chunk's products; 0: read, multiply, read.
+#ifndef MXB_XY_PRELOAD
+#define MXB_XY_PRELOAD 1
+#endif
 static_assert((MXB_XY_SWZ & kXYSwz) == 0 && MXB_XY_SWZ <= 16, "swizzle group: a power of two up to 16");
 
 template <int RT, int TX, int D, bool SPLIT, int FIN>
@@ -125,6 +132,21 @@ struct XYShape {
     static_assert(D >= 1 && (D - 1) * GL + NST * ST < 64, "vmcnt range");
     static constexpr bool FITS = READS_IN_ROW && BYTES <= 160 * 1024;  // gfx950 LDS per workgroup
 };
+
+// x-pass products of operand read q: it is chunk c = q - i of tile t0 + 2 i
+// (taps in chunk order per tile).
+template <class L, int NT, int C, bool SPLIT, int NR>
+__device__ __forceinline__ void xy_x_products(int q, const v8s (&xs)[NR], const v8s (&xls)[NR], const v8s (&ahi)[C],
+                                              const v8s (&alo)[C], v4f (&acc)[NT]) {
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+        const int c = q - i;
+        if (c < 0 || c >= C) continue;
+        acc[i] = L::mfma(ahi[c], xs[q], acc[i]);
+        acc[i] = L::mfma(alo[c], xs[q], acc[i]);
+        if constexpr (SPLIT) acc[i] = L::mfma(ahi[c], xls[q], acc[i]);
+    }
+}
 
 // LDS ordering between the lanes of one wave: its LDS operations execute in
 // order; the fences keep the compiler from moving them across.
@@ -191,7 +213,7 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
                                                   TapsL<2 * RT + 1> taps, SplitIO io) {
     using S = XYShape<RT, TX, D, SPLIT, FIN>;
     using L = Lowp<BF>;
-    constexpr int P = S::P, C = S::C, W = S::W, PR = S::PR, SB = S::SB, NI = S::NI, NSUB = S::NSUB;
+    constexpr int P = S::P, C = S::C, W = S::W, PR = S::PR, SB = S::SB, NI = S::NI;
     constexpr int SLOT = S::SLOT, MW = S::MW, MSLOT = S::MSLOT, MHALF = S::MHALF, ST = S::ST, GL = S::GL;
     constexpr int TW = LowpChunks<RT>::TW;
     static_assert(!SPLIT || !BF, "the split form runs on FP16 MFMAs");
@@ -335,6 +357,7 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
             v4f acc[NT];
 #pragma unroll
             for (int i = 0; i < NT; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+            v8s xs[NR], xls[NR];
 #pragma unroll
             for (int q = 0; q < NR; ++q) {
                 // halves [o, o + 8) of the staged row
@@ -367,15 +390,14 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
                     if constexpr (SPLIT)
                         xl = __builtin_bit_cast(v8s, make_uint4(plo[0].x, plo[0].y, plo[1].x, plo[1].y));
                 }
-                // read q is chunk c = q - i of tile t0 + 2 i (taps in chunk order per tile)
+                xs[q] = x;
+                if constexpr (SPLIT) xls[q] = xl;
+                if constexpr (!MXB_XY_PRELOAD) xy_x_products<L, NT, C, SPLIT>(q, xs, xls, ahi, alo, acc);
+            }
+            if constexpr (MXB_XY_PRELOAD) {
+                __builtin_amdgcn_sched_barrier(0);  // keeps the reads ahead of the products
 #pragma unroll
-                for (int i = 0; i < NT; ++i) {
-                    const int c = q - i;
-                    if (c < 0 || c >= C) continue;
-                    acc[i] = L::mfma(ahi[c], x, acc[i]);
-                    acc[i] = L::mfma(alo[c], x, acc[i]);
-                    if constexpr (SPLIT) acc[i] = L::mfma(ahi[c], xl, acc[i]);
-                }
+                for (int q = 0; q < NR; ++q) xy_x_products<L, NT, C, SPLIT>(q, xs, xls, ahi, alo, acc);
             }
 #pragma unroll
             for (int i = 0; i < NT; ++i) {
@@ -411,34 +433,51 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
         for (int gq = 0; gq < NQ; ++gq)
 #pragma unroll
             for (int u = 0; u < 2; ++u) acc[gq][u] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        // operands of chunk c for group gq and row tile u: two transposed
+        // 8-byte reads per plane (rows rl and rl + 4 of the slots)
+        v8s yx[C][NQ][2], yxl[C][NQ][2];
+        auto y_read = [&](int c, int gq, int u) {
+            const int sb = (wv >> 2) + 2 * gq;  // sub-ring
+            typedef __attribute__((address_space(3))) v4s lds_v4s;
+            v4s part[2], plo[2];
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                int si = sp + c + radj[u][hh];
+                si = si >= W ? si - W : si;
+                si = si >= W ? si - W : si;
+                const unsigned char *a = ring + si * SLOT + sb * kSlotBytes + roff[u][hh];
+                part[hh] = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)a);
+                if constexpr (SPLIT) plo[hh] = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)(a + W * SLOT));
+            }
+            yx[c][gq][u] = v8s{part[0][0], part[0][1], part[0][2], part[0][3],
+                               part[1][0], part[1][1], part[1][2], part[1][3]};
+            if constexpr (SPLIT)
+                yxl[c][gq][u] = v8s{plo[0][0], plo[0][1], plo[0][2], plo[0][3],
+                                    plo[1][0], plo[1][1], plo[1][2], plo[1][3]};
+        };
+        auto y_products = [&](int c, int gq, int u) {
+            acc[gq][u] = L::mfma(ahi[c], yx[c][gq][u], acc[gq][u]);
+            acc[gq][u] = L::mfma(alo[c], yx[c][gq][u], acc[gq][u]);
+            if constexpr (SPLIT) acc[gq][u] = L::mfma(ahi[c], yxl[c][gq][u], acc[gq][u]);
+        };
 #pragma unroll
         for (int c = 0; c < C; ++c)
 #pragma unroll
             for (int gq = 0; gq < NQ; ++gq)
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
-                    const int sb = (wv >> 2) + 2 * gq;  // sub-ring
-                    typedef __attribute__((address_space(3))) v4s lds_v4s;
-                    v4s part[2], plo[2];
-#pragma unroll
-                    for (int hh = 0; hh < 2; ++hh) {
-                        int si = sp + c + radj[u][hh];
-                        si = si >= W ? si - W : si;
-                        si = si >= W ? si - W : si;
-                        const unsigned char *a = ring + si * SLOT + sb * kSlotBytes + roff[u][hh];
-                        part[hh] = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)a);
-                        if constexpr (SPLIT) plo[hh] = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)(a + W * SLOT));
-                    }
-                    const v8s x = {part[0][0], part[0][1], part[0][2], part[0][3],
-                                   part[1][0], part[1][1], part[1][2], part[1][3]};
-                    acc[gq][u] = L::mfma(ahi[c], x, acc[gq][u]);
-                    acc[gq][u] = L::mfma(alo[c], x, acc[gq][u]);
-                    if constexpr (SPLIT) {
-                        const v8s xl = {plo[0][0], plo[0][1], plo[0][2], plo[0][3],
-                                        plo[1][0], plo[1][1], plo[1][2], plo[1][3]};
-                        acc[gq][u] = L::mfma(ahi[c], xl, acc[gq][u]);
-                    }
+                    y_read(c, gq, u);
+                    if constexpr (!MXB_XY_PRELOAD) y_products(c, gq, u);
                 }
+        if constexpr (MXB_XY_PRELOAD) {
+            __builtin_amdgcn_sched_barrier(0);  // keeps the reads ahead of the products
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+#pragma unroll
+                for (int gq = 0; gq < NQ; ++gq)
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) y_products(c, gq, u);
+        }
 
         const unsigned char *const msl = mring + (p % (D + 1)) * MSLOT;
         if constexpr (S::WEPI) {
