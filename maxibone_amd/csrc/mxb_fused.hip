@@ -68,11 +68,23 @@ constexpr int kXYSwz = MXB_XY_SWZ - 1;
 #define MXB_XY_WAVE_EPI 1
 #endif
 
-// 1: each pass issues all of a wave's operand reads for the step (x-pass) or
-// pair (y-pass) before its MFMAs, so the LDS latencies overlap instead of
-// each read waiting on the previous chunk's products; 0: read, multiply, read.
-#ifndef MXB_XY_PRELOAD
-#define MXB_XY_PRELOAD 1
+// Per-form shape of the 16-bit and split kernels (measured on 1024^3, R=40,
+// ms per fused pass):
+// * split form: its own epilogue stage, 3 steps of prefetch, one block per
+//   CU (158 KB of LDS), operand reads issued before the MFMAs (2.43 against
+//   2.52 interleaved);
+// * 16-bit forms: the epilogue stages in the staging buffer its own step has
+//   just consumed (the x-pass read it before the y-pass barrier; the next
+//   step's loads refill it only after that step's first barrier), 1 step of
+//   prefetch, so the block needs 51 KB and three blocks share a CU (six waves
+//   per SIMD, at most 85 VGPRs; the early operand reads would spill there):
+//   1.26 against 1.35 with 3 steps at two blocks per CU.
+// MXB_XY_PRELOAD forces the operand-read order (1 early, 0 interleaved) for
+// both forms.
+#ifdef MXB_XY_PRELOAD
+#define MXB_XY_PRELOAD_FOR(split) (MXB_XY_PRELOAD)
+#else
+#define MXB_XY_PRELOAD_FOR(split) (split)
 #endif
 static_assert((MXB_XY_SWZ & kXYSwz) == 0 && MXB_XY_SWZ <= 16, "swizzle group: a power of two up to 16");
 
@@ -98,13 +110,20 @@ struct XYShape {
     static constexpr int PLO = (HALO - (RT + 7) / 8 * 8) / 8;  // first piece the radius needs
     static constexpr int STAGE = (D + 1) * P * SB;    // D steps in flight while one computes
     static constexpr int RING = W * P * SLOT;
-    static constexpr int SPITCH16 = 2 * TX + 16;      // epilogue stage pitch (bytes), 16-bit
-    static constexpr int SPITCHF = 4 * TX + 16;       // epilogue stage pitch (bytes), FP32
+    // the block epilogue's stage: the consumed staging buffer (16-bit forms;
+    // unpadded rows, 16-byte pieces swizzled by row) or an area of its own
+    static constexpr bool ALIAS = !SPLIT;
+    static constexpr bool PRELOAD = MXB_XY_PRELOAD_FOR(SPLIT);
+    static constexpr int SPITCH16 = 2 * TX + (ALIAS ? 0 : 16);  // block stage pitch (bytes), 16-bit
+    static constexpr int SPITCHF = 4 * TX + (ALIAS ? 0 : 16);   // block stage pitch (bytes), FP32
     static constexpr int WP16 = 48;                   // wave stage pitch (bytes), 16-bit: 16 columns + pad
     static constexpr int WPF = 80;                    // wave stage pitch (bytes), FP32
     static constexpr int WEST = 32 * (FIN != 0 ? WPF : P * WP16);  // wave stage bytes
     static constexpr bool WEPI = MXB_XY_WAVE_EPI == 2 || (MXB_XY_WAVE_EPI == 1 && SPLIT);
-    static constexpr int EST = WEPI ? NW * WEST : 32 * (FIN != 0 ? SPITCHF : P * SPITCH16);
+    static constexpr int EST_NEED = WEPI ? NW * WEST : 32 * (FIN != 0 ? SPITCHF : P * SPITCH16);
+    static constexpr int EST = ALIAS && !WEPI ? 0 : EST_NEED;
+    static_assert(!(ALIAS && !WEPI) || (EST_NEED <= P * SB && (2 * TX / 16) % 16 == 0),
+                  "the block stage fits one step's staging buffer in whole 16-piece swizzle groups");
     static constexpr int MRING = (D + 1) * MSLOT + 256;  // + the target of the idle waves' mask loads
     static constexpr int TAPS = 2 * LowpChunks<RT>::TW * 2;
     static constexpr int BYTES = STAGE + RING + EST + MRING + TAPS;
@@ -146,6 +165,16 @@ __device__ __forceinline__ void xy_x_products(int q, const v8s (&xs)[NR], const 
         acc[i] = L::mfma(alo[c], xs[q], acc[i]);
         if constexpr (SPLIT) acc[i] = L::mfma(ahi[c], xls[q], acc[i]);
     }
+}
+
+// Byte offset of byte b of row r in the block epilogue's stage of
+// `pitch`-byte rows: padded rows of its own, or (aliased into a staging
+// buffer) unpadded rows whose 16-byte pieces are XOR-swizzled by the row, so
+// that the y-pass lanes' writes 4 rows apart fall in distinct banks.
+template <bool ALIAS>
+__device__ __forceinline__ int stage_off(int r, int b, int pitch) {
+    if constexpr (ALIAS) return r * pitch + ((((b >> 4) ^ (r & 15)) << 4) | (b & 15));
+    else return r * pitch + b;
 }
 
 // LDS ordering between the lanes of one wave: its LDS operations execute in
@@ -208,7 +237,7 @@ __device__ __forceinline__ void xy_store_state(uint4 v, uint4 vl, uint32_t m, bo
 // that completes slot p + W - 1.  Its mask words come with steps 2 p + 2 W - 2
 // (rows 0 .. 15) and 2 p + 2 W - 1 (rows 16 .. 31).
 template <int RT, int TX, int D, bool BF, bool SPLIT, int FIN>
-__global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ in, uint16_t *__restrict__ out,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT ? 1 : 6))) void k_xy_fused(const uint16_t *__restrict__ in, uint16_t *__restrict__ out,
                                                   const uint32_t *__restrict__ bits, XYGeom g,
                                                   TapsL<2 * RT + 1> taps, SplitIO io) {
     using S = XYShape<RT, TX, D, SPLIT, FIN>;
@@ -221,8 +250,8 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
     __shared__ __attribute__((aligned(16))) unsigned char smem[S::BYTES];
     unsigned char *const staging = smem;
     unsigned char *const ring = smem + S::STAGE;
-    unsigned char *const est = ring + S::RING;
-    unsigned char *const mring = est + S::EST;
+    unsigned char *const est_own = ring + S::RING;  // the epilogue stage when not aliased
+    unsigned char *const mring = est_own + S::EST;
     uint16_t *const thi = reinterpret_cast<uint16_t *>(mring + S::MRING);
     uint16_t *const tlo = thi + TW;
 
@@ -340,7 +369,10 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
             vm_wait<(D - 1) * GL + S::NST * ST>();
         __syncthreads();
         issue_group(j + D);
-        const unsigned char *const stg = staging + (j % (D + 1)) * P * SB;
+        unsigned char *const stg = staging + (j % (D + 1)) * P * SB;
+        // the block epilogue of this (odd) step stages in stg when aliased:
+        // every wave's x-pass reads of it precede the y-pass barrier
+        unsigned char *const est = S::ALIAS && !S::WEPI ? stg : est_own;
 
         // ---- x-pass of the step's 16 rows.  Wave wv takes the column tiles
         // t0 + 2 i (i < NT), t0 = (wv & 1) + 2 NT (wv >> 1): tile t + 2
@@ -392,9 +424,9 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
                 }
                 xs[q] = x;
                 if constexpr (SPLIT) xls[q] = xl;
-                if constexpr (!MXB_XY_PRELOAD) xy_x_products<L, NT, C, SPLIT>(q, xs, xls, ahi, alo, acc);
+                if constexpr (!S::PRELOAD) xy_x_products<L, NT, C, SPLIT>(q, xs, xls, ahi, alo, acc);
             }
-            if constexpr (MXB_XY_PRELOAD) {
+            if constexpr (S::PRELOAD) {
                 __builtin_amdgcn_sched_barrier(0);  // keeps the reads ahead of the products
 #pragma unroll
                 for (int q = 0; q < NR; ++q) xy_x_products<L, NT, C, SPLIT>(q, xs, xls, ahi, alo, acc);
@@ -467,9 +499,9 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
                     y_read(c, gq, u);
-                    if constexpr (!MXB_XY_PRELOAD) y_products(c, gq, u);
+                    if constexpr (!S::PRELOAD) y_products(c, gq, u);
                 }
-        if constexpr (MXB_XY_PRELOAD) {
+        if constexpr (S::PRELOAD) {
             __builtin_amdgcn_sched_barrier(0);  // keeps the reads ahead of the products
 #pragma unroll
             for (int c = 0; c < C; ++c)
@@ -540,14 +572,17 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
                     const int row = 16 * u + 4 * kg + e;
                     const int xc = 64 * ((wv >> 2) + 2 * gq) + 16 * quad + col;
                     if constexpr (FIN != 0) {
-                        *reinterpret_cast<float *>(est + row * S::SPITCHF + 4 * xc) = acc[gq][u][e] * io.scale;
+                        *reinterpret_cast<float *>(est + stage_off<S::ALIAS>(row, 4 * xc, S::SPITCHF)) =
+                            acc[gq][u][e] * io.scale;
                     } else if constexpr (SPLIT) {
                         uint16_t hv, lv;
                         split_f16(acc[gq][u][e] * io.scale, hv, lv);
-                        *reinterpret_cast<uint16_t *>(est + row * S::SPITCH16 + 2 * xc) = hv;
-                        *reinterpret_cast<uint16_t *>(est + 32 * S::SPITCH16 + row * S::SPITCH16 + 2 * xc) = lv;
+                        const int so = stage_off<S::ALIAS>(row, 2 * xc, S::SPITCH16);
+                        *reinterpret_cast<uint16_t *>(est + so) = hv;
+                        *reinterpret_cast<uint16_t *>(est + 32 * S::SPITCH16 + so) = lv;
                     } else {
-                        *reinterpret_cast<uint16_t *>(est + row * S::SPITCH16 + 2 * xc) = L::from_f32(acc[gq][u][e]);
+                        *reinterpret_cast<uint16_t *>(est + stage_off<S::ALIAS>(row, 2 * xc, S::SPITCH16)) =
+                            L::from_f32(acc[gq][u][e]);
                     }
                 }
         __syncthreads();
@@ -567,13 +602,13 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
                 v4f v[2];
 #pragma unroll
                 for (int hh = 0; hh < 2; ++hh)
-                    v[hh] = *reinterpret_cast<const v4f *>(est + rw * S::SPITCHF + 32 * pc + 16 * hh);
+                    v[hh] = *reinterpret_cast<const v4f *>(est + stage_off<S::ALIAS>(rw, 32 * pc + 16 * hh, S::SPITCHF));
                 xy_store_fin<FIN>(v, m, ok, off, io, sinkf, sink);
             } else {
-                const uint4 v = *reinterpret_cast<const uint4 *>(est + rw * S::SPITCH16 + 16 * pc);
+                const int so = stage_off<S::ALIAS>(rw, 16 * pc, S::SPITCH16);
+                const uint4 v = *reinterpret_cast<const uint4 *>(est + so);
                 uint4 vl = make_uint4(0, 0, 0, 0);
-                if constexpr (SPLIT)
-                    vl = *reinterpret_cast<const uint4 *>(est + 32 * S::SPITCH16 + rw * S::SPITCH16 + 16 * pc);
+                if constexpr (SPLIT) vl = *reinterpret_cast<const uint4 *>(est + 32 * S::SPITCH16 + so);
                 xy_store_state<BF, SPLIT>(v, vl, m, ok, off, out, io, sink);
             }
         }
@@ -587,7 +622,7 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
 #define MXB_XY_TX16 128
 #endif
 #ifndef MXB_XY_D16
-#define MXB_XY_D16 3
+#define MXB_XY_D16 1
 #endif
 #ifndef MXB_XY_TXS
 #define MXB_XY_TXS 128
@@ -597,13 +632,14 @@ __global__ __launch_bounds__(512) void k_xy_fused(const uint16_t *__restrict__ i
 #endif
 
 // Strip width and prefetch depth of a form and radius bucket: the deepest
-// prefetch up to the form's default whose shape fits the LDS (the split form
-// at bucket 48 takes 2, at bucket 64 1).  FITS: the fused pass serves the
-// bucket.  The 16-bit forms take it only where two blocks share a CU (80 KB
-// of LDS each; buckets up to 40): at buckets 48 and 64 one block per CU made
-// it slower than the three-pass form (1024^3 BF16, sigma 12: 251.7 against
-// 260.0 Gvoxel*iter/s; sigma 15: 232.4 against 241.3), while the split form
-// gains there (195.2 against 159.8; 160.6 against 135.4).
+// prefetch up to the form's default whose shape fits the LDS.  FITS: the
+// fused pass serves the bucket.  The 16-bit forms take it where two blocks
+// share a CU (80 KB of LDS each): with one block per CU it was slower than
+// the three-pass form at buckets 48 and 64 (1024^3 BF16, sigma 12: 251.7
+// against 260.0 Gvoxel*iter/s); with the aliased stage and one step of
+// prefetch every bucket fits two or more, and sigma 12 / 15 run 281.0 / 256.0
+// against 260.6 / 240.7.  The split form gains at every bucket (sigma 12:
+// 195.2 against 159.8; sigma 15: 160.6 against 135.4).
 template <int RT, bool SPLIT>
 struct XYForm {
     static constexpr int TX = SPLIT ? MXB_XY_TXS : MXB_XY_TX16;
