@@ -314,11 +314,28 @@ __global__ __launch_bounds__(256) void k_line_generic(const float *__restrict__ 
 }
 
 // ------------------------------------------------------------ element-wise
+// u = (m > 0) as float (cpu:243-248); 8 elements per thread (an 8-byte mask
+// load, two 16-byte stores) when `vec` says the pointers allow it, then the
+// tail (or everything) one element per thread.
 __global__ __launch_bounds__(256) void k_init_state(const uint8_t *__restrict__ m,
-                                                    float *__restrict__ u, int64_t n) {
+                                                    float *__restrict__ u, int64_t n, int vec) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        u[i] = m[i] > 0 ? 1.0f : 0.0f;  // cpu:243-248
+    int64_t done = 0;
+    if (vec) {
+        const int64_t n8 = n >> 3;
+        for (int64_t g = tid; g < n8; g += stride) {
+            const uint2 mv = *reinterpret_cast<const uint2 *>(m + 8 * g);
+            float r[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) r[e] = (((e < 4 ? mv.x : mv.y) >> (8 * (e & 3))) & 0xffu) ? 1.0f : 0.0f;
+            float4 *dst = reinterpret_cast<float4 *>(u + 8 * g);
+            dst[0] = make_float4(r[0], r[1], r[2], r[3]);
+            dst[1] = make_float4(r[4], r[5], r[6], r[7]);
+        }
+        done = n8 << 3;
+    }
+    for (int64_t i = done + tid; i < n; i += stride) u[i] = m[i] > 0 ? 1.0f : 0.0f;
 }
 
 __global__ __launch_bounds__(256) void k_quantise(const float *__restrict__ u,
@@ -349,7 +366,9 @@ static inline int64_t grid_stride_blocks(int64_t n) {
 
 hipError_t launch_init_state(const uint8_t *mask, float *u, int64_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_init_state, dim3((unsigned)grid_stride_blocks(n)), dim3(256), 0, s, mask, u, n);
+    const int vec = reinterpret_cast<uintptr_t>(mask) % 8 == 0 && reinterpret_cast<uintptr_t>(u) % 16 == 0;
+    hipLaunchKernelGGL(k_init_state, dim3((unsigned)grid_stride_blocks(vec ? (n + 7) / 8 : n)), dim3(256), 0, s,
+                       mask, u, n, vec);
     return hipGetLastError();
 }
 
@@ -365,35 +384,50 @@ hipError_t launch_quantise(const float *u, uint16_t *q, float *f, int64_t n, hip
     return hipGetLastError();
 }
 
-// Pack a mask into bits, words interleaved across rows (the MaskBits layout):
-// one wave per 64 positions of a row; each lane tests one byte and a ballot
-// collects the two words.
+// Pack a mask into bits, words interleaved across rows (the MaskBits layout).
+// Thread (y, w, z) builds word w of row (z, y) from its 32 bytes, so the
+// threads of a block store consecutive words (consecutive y); each reads one
+// whole 32-byte run.  Rows whose runs are 16-byte aligned take two 16-byte
+// loads; the rest, and a row's partial last word, take bytes.
+
+// Bit i of the result: byte i of v is nonzero (i = 0 .. 3).
+__device__ __forceinline__ uint32_t nonzero_bytes4(uint32_t v) {
+    const uint32_t t = (((v & 0x7f7f7f7fu) + 0x7f7f7f7fu) | v) & 0x80808080u;
+    return ((t >> 7) & 1u) | ((t >> 14) & 2u) | ((t >> 21) & 4u) | ((t >> 28) & 8u);
+}
+
 __global__ __launch_bounds__(256) void k_pack_mask_bits(const uint8_t *__restrict__ m, int64_t nz, int64_t ny,
                                                         int64_t nx, int64_t mask_row, int64_t mask_plane,
                                                         uint32_t *__restrict__ bits) {
-    const int64_t wpr = (nx + 31) / 32, cpr = (nx + 63) / 64;
-    const int64_t total = nz * ny * cpr;
-    const int lane = threadIdx.x & 63;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < total; i += nwaves) {
-        const int64_t c = i % cpr, row = i / cpr;
-        const int64_t y = row % ny, z = row / ny;
-        const int64_t x = 64 * c + lane;
-        const bool on = x < nx && m[z * mask_plane + y * mask_row + x] != 0;
-        const unsigned long long b = __ballot(on);
-        const int64_t w = 2 * c + lane;
-        if (lane < 2 && w < wpr) bits[(z * wpr + w) * ny + y] = (uint32_t)(b >> (32 * lane));
+    const int64_t y = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t w = blockIdx.y;
+    const int64_t wpr = (nx + 31) / 32;
+    if (y >= ny) return;
+    const int64_t x0 = 32 * w;
+    for (int64_t z = blockIdx.z; z < nz; z += gridDim.z) {
+        const uint8_t *row = m + z * mask_plane + y * mask_row + x0;
+        uint32_t word = 0;
+        if (x0 + 32 <= nx && (reinterpret_cast<uintptr_t>(row) & 15) == 0) {
+            const uint4 a = *reinterpret_cast<const uint4 *>(row);
+            const uint4 b = *reinterpret_cast<const uint4 *>(row + 16);
+            const uint32_t d[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int k = 0; k < 8; ++k) word |= nonzero_bytes4(d[k]) << (4 * k);
+        } else {
+            const int n = (int)(nx - x0 < 32 ? nx - x0 : 32);
+            for (int i = 0; i < n; ++i) word |= (row[i] != 0 ? 1u : 0u) << i;
+        }
+        bits[(z * wpr + w) * ny + y] = word;
     }
 }
 
 hipError_t launch_pack_mask_bits(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx, int64_t mask_row,
                                  int64_t mask_plane, uint32_t *bits, hipStream_t s) {
-    const int64_t chunks = nz * ny * ((nx + 63) / 64);  // one wave each
-    if (chunks == 0) return hipSuccess;
-    int64_t blocks = (chunks + 3) / 4;
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_pack_mask_bits, dim3((unsigned)blocks), dim3(256), 0, s, mask, nz, ny, nx, mask_row,
-                       mask_plane, bits);
+    if (nz * ny * nx == 0) return hipSuccess;
+    const int64_t wpr = (nx + 31) / 32;
+    if (wpr > 65535) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((ny + 255) / 256), (unsigned)wpr, (unsigned)(nz < 4096 ? nz : 4096));
+    hipLaunchKernelGGL(k_pack_mask_bits, grid, dim3(256), 0, s, mask, nz, ny, nx, mask_row, mask_plane, bits);
     return hipGetLastError();
 }
 

@@ -148,28 +148,78 @@ __global__ __launch_bounds__(256) void k_line_lowp(const uint16_t *__restrict__ 
     }
 }
 
+// The element-wise kernels below take 8 elements per thread (an 8-byte mask
+// load, 16-byte stores) when `vec` says the pointers allow it, and finish the
+// tail, or everything when they do not, one element per thread.
+
+// The 16-bit values (v0, v1) of a byte pair of w at bit `sh`: `one` where the
+// byte is nonzero, else 0.
+__device__ __forceinline__ uint32_t mask_pair16(uint32_t w, int sh, uint16_t one) {
+    return (((w >> sh) & 0xffu) ? (uint32_t)one : 0u) | ((((w >> (sh + 8)) & 0xffu) ? (uint32_t)one : 0u) << 16);
+}
+
+__device__ __forceinline__ uint4 mask8_to16(uint2 mv, uint16_t one) {
+    return make_uint4(mask_pair16(mv.x, 0, one), mask_pair16(mv.x, 16, one), mask_pair16(mv.y, 0, one),
+                      mask_pair16(mv.y, 16, one));
+}
+
 template <bool BF>
 __global__ __launch_bounds__(256) void k_init_lowp(const uint8_t *__restrict__ m,
-                                                   uint16_t *__restrict__ u, int64_t n) {
+                                                   uint16_t *__restrict__ u, int64_t n, int vec) {
     const uint16_t one = Lowp<BF>::from_f32(1.0f);
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        u[i] = m[i] > 0 ? one : (uint16_t)0;
+    int64_t done = 0;
+    if (vec) {
+        const int64_t n8 = n >> 3;
+        for (int64_t g = tid; g < n8; g += stride)
+            *reinterpret_cast<uint4 *>(u + 8 * g) = mask8_to16(*reinterpret_cast<const uint2 *>(m + 8 * g), one);
+        done = n8 << 3;
+    }
+    for (int64_t i = done + tid; i < n; i += stride) u[i] = m[i] > 0 ? one : (uint16_t)0;
 }
 
 template <bool BF>
 __global__ __launch_bounds__(256) void k_lowp_to_f32(const uint16_t *__restrict__ u,
-                                                     float *__restrict__ f, int64_t n) {
+                                                     float *__restrict__ f, int64_t n, int vec) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        f[i] = Lowp<BF>::to_f32(u[i]);
+    int64_t done = 0;
+    if (vec) {
+        const int64_t n8 = n >> 3;
+        for (int64_t g = tid; g < n8; g += stride) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(u + 8 * g);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+            float r[8];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                r[2 * k] = Lowp<BF>::to_f32((uint16_t)(w[k] & 0xffffu));
+                r[2 * k + 1] = Lowp<BF>::to_f32((uint16_t)(w[k] >> 16));
+            }
+            float4 *dst = reinterpret_cast<float4 *>(f + 8 * g);
+            dst[0] = make_float4(r[0], r[1], r[2], r[3]);
+            dst[1] = make_float4(r[4], r[5], r[6], r[7]);
+        }
+        done = n8 << 3;
+    }
+    for (int64_t i = done + tid; i < n; i += stride) f[i] = Lowp<BF>::to_f32(u[i]);
 }
 
 // Split-form planes from the mask: hi = f16(1) on the mask, else 0; lo = 0.
 __global__ __launch_bounds__(256) void k_init_split(const uint8_t *__restrict__ m, uint16_t *__restrict__ p,
-                                                    int64_t lo_off, int64_t n) {
+                                                    int64_t lo_off, int64_t n, int vec) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    int64_t done = 0;
+    if (vec) {
+        const int64_t n8 = n >> 3;
+        for (int64_t g = tid; g < n8; g += stride) {
+            *reinterpret_cast<uint4 *>(p + 8 * g) = mask8_to16(*reinterpret_cast<const uint2 *>(m + 8 * g), 0x3c00);
+            *reinterpret_cast<uint4 *>(p + lo_off + 8 * g) = make_uint4(0u, 0u, 0u, 0u);
+        }
+        done = n8 << 3;
+    }
+    for (int64_t i = done + tid; i < n; i += stride) {
         p[i] = m[i] > 0 ? (uint16_t)0x3c00 : (uint16_t)0;
         p[lo_off + i] = 0;
     }
@@ -249,21 +299,31 @@ static inline int64_t blocks_for(int64_t n) {
     return b < 1 ? 1 : (b > 65536 ? 65536 : b);
 }
 
+// Whether every pointer is aligned to its vector width (mask 8, the rest 16).
+static inline bool vec_ok(const void *mask8, const void *a16, const void *b16 = nullptr) {
+    return reinterpret_cast<uintptr_t>(mask8) % 8 == 0 && reinterpret_cast<uintptr_t>(a16) % 16 == 0 &&
+           reinterpret_cast<uintptr_t>(b16) % 16 == 0;
+}
+
 hipError_t launch_init_lowp(const uint8_t *mask, uint16_t *u, int64_t n, bool bf, hipStream_t s) {
     if (n == 0) return hipSuccess;
+    const int vec = vec_ok(mask, u);
+    const dim3 grid((unsigned)blocks_for(vec ? (n + 7) / 8 : n));
     if (bf)
-        hipLaunchKernelGGL(k_init_lowp<true>, dim3((unsigned)blocks_for(n)), dim3(256), 0, s, mask, u, n);
+        hipLaunchKernelGGL(k_init_lowp<true>, grid, dim3(256), 0, s, mask, u, n, vec);
     else
-        hipLaunchKernelGGL(k_init_lowp<false>, dim3((unsigned)blocks_for(n)), dim3(256), 0, s, mask, u, n);
+        hipLaunchKernelGGL(k_init_lowp<false>, grid, dim3(256), 0, s, mask, u, n, vec);
     return hipGetLastError();
 }
 
 hipError_t launch_lowp_to_f32(const uint16_t *u, float *f, int64_t n, bool bf, hipStream_t s) {
     if (n == 0) return hipSuccess;
+    const int vec = vec_ok(nullptr, u, f);
+    const dim3 grid((unsigned)blocks_for(vec ? (n + 7) / 8 : n));
     if (bf)
-        hipLaunchKernelGGL(k_lowp_to_f32<true>, dim3((unsigned)blocks_for(n)), dim3(256), 0, s, u, f, n);
+        hipLaunchKernelGGL(k_lowp_to_f32<true>, grid, dim3(256), 0, s, u, f, n, vec);
     else
-        hipLaunchKernelGGL(k_lowp_to_f32<false>, dim3((unsigned)blocks_for(n)), dim3(256), 0, s, u, f, n);
+        hipLaunchKernelGGL(k_lowp_to_f32<false>, grid, dim3(256), 0, s, u, f, n, vec);
     return hipGetLastError();
 }
 
@@ -273,7 +333,9 @@ namespace mxb {
 
 hipError_t launch_init_split(const uint8_t *mask, uint16_t *planes, int64_t lo_off, int64_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_init_split, dim3((unsigned)blocks_for(n)), dim3(256), 0, s, mask, planes, lo_off, n);
+    const int vec = vec_ok(mask, planes, planes + lo_off);
+    hipLaunchKernelGGL(k_init_split, dim3((unsigned)blocks_for(vec ? (n + 7) / 8 : n)), dim3(256), 0, s, mask,
+                       planes, lo_off, n, vec);
     return hipGetLastError();
 }
 
