@@ -9,29 +9,32 @@
 // rounding; illumination (cpu:295-302) follows the last pass, as there.
 //
 // Block = 512 threads (8 waves) on plane z, output columns [x0, x0 + TX) and
-// output rows [ya, yb).  It marches along y in steps of 32 rows:
-//   1. direct loads bring input rows ya - RT + 32 j .. +31, columns
-//      [x0 - HALO, x0 + TX + HALO), into a staging buffer (two buffers: step
-//      j + 1 loads while step j computes); pieces outside the halo the radius
-//      needs, and everything outside the volume, read a zero block;
+// output rows [ya, yb).  It marches along y in steps of 16 rows:
+//   1. direct loads bring input rows ya - RT + 16 j .. +15, columns
+//      [x0 - HALO, x0 + TX + HALO), into a staging buffer (D + 1 buffers:
+//      steps j + 1 .. j + D load while step j computes); pieces outside the
+//      halo the radius needs, and everything outside the volume, read a zero
+//      block;
 //   2. the x-pass: 16x16x32 MFMA tiles with lines = rows and positions = x,
 //      operands read straight from the staged rows (no transpose), results
-//      rounded to the state format and written into slot j of a y-ring of
-//      W slots laid out as the 16-bit body's slots (32 positions x 64 lines
-//      per 4 KB, one per 64-column sub-ring);
-//   3. the y-pass of output pair p = j - W + 1 from slots p .. j with the
-//      16-bit body's transposed operand reads (lines = x, positions = y);
+//      rounded to the state format and written into half j % 2 of slot
+//      j / 2 of a y-ring of W slots laid out as the 16-bit body's slots (32
+//      positions x 64 lines per 4 KB, one per 64-column sub-ring);
+//   3. after every second step, the y-pass of the next 32-row output pair
+//      from W slots with the 16-bit body's transposed operand reads (lines =
+//      x, positions = y);
 //   4. the epilogue stages the pair's 32 rows x TX outputs in LDS, sets mask
 //      voxels to 1 from the mask bits (direct-loaded every second step), and
 //      writes whole 16-byte row pieces.
 // Arithmetic, tap split and tile chunking are the 16-bit body's (mxb_lowp.h);
 // the split form (FP16 hi + lo planes, SplitIO) runs three MFMAs per chunk.
 //
-// Wait counts: every direct load of step j + 1 is issued after the barrier
-// that follows the wait for step j, and the only vector-memory operations
-// between two waits are one step's loads and one epilogue's ST stores per
-// thread (all unconditional: stores outside the volume go to g_dma_sink), so
-// the wait before step j is vmcnt(ST) once an epilogue ran, else vmcnt(0).
+// Wait counts: the loads of step j + D are issued after the barrier that
+// follows the wait for step j, and the only vector-memory operations are the
+// steps' loads (GL per thread) and the epilogues' stores (ST per thread, all
+// unconditional: stores outside the volume go to g_dma_sink), so the wait
+// before step j leaves outstanding the loads of steps j + 1 .. j + D - 1 and
+// the stores of the epilogues that ran after step j's loads were issued.
 #include "maxibone_diffusion.h"
 #include "mxb_dma.h"
 #include "mxb_lowp.h"
