@@ -42,7 +42,7 @@ PRECISION_NOTE = {
                   "fused x->y pass; error against the FP32 reference path and FP64 in error_vs_fp64",
     "fp32": "plain FP32: FP32 state, FP32 MFMA chain in tap order (equal to one FMA per tap); "
             "z, y, x passes",
-    "bf16": "BF16 state, BF16 MFMA passes (z-pass plus fused x->y), FP32 last repetition",
+    "bf16": "BF16 state, BF16 MFMA passes (z-pass plus fused x->y), last repetition in the FP32-class split form",
 }
 
 

@@ -34,8 +34,13 @@ typedef enum {
 /* Storage / arithmetic form of the diffusion state. */
 typedef enum {
     MXB_FP32 = 0,       /* FP32 state, FP32 banded-Toeplitz MFMA passes          */
-    MXB_BF16 = 1,       /* BF16 state, BF16 MFMA passes, FP32 final repetition    */
-    MXB_FP16 = 2,       /* FP16 state, FP16 MFMA passes, FP32 final repetition    */
+    MXB_BF16 = 1,       /* BF16 state, BF16 MFMA passes, FP32-accuracy final
+                           repetition (see below)                               */
+    MXB_FP16 = 2,       /* FP16 state, FP16 MFMA passes, FP32-accuracy final
+                           repetition: in the MXB_FP32_SPLIT form where its
+                           fused x -> y pass serves the volume (x extent a
+                           multiple of 8), else in MXB_FP32; a single
+                           repetition runs in MXB_FP32                          */
     MXB_FP32_FMA = 3,   /* FP32 state, one VALU FMA per tap                       */
     MXB_FP32_EXACT = 4, /* FP32 state, product-then-add: bit-equal to the
                            reference CPU build                                  */
@@ -81,7 +86,8 @@ int mxb_diffusion_ex(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
  * a halo of temporal*R layers (temporal = 1 is the reference's
  * one-step-per-chunk scheme, cpu:433-453 / gpu:869-905).  Results equal the
  * in-memory form of the same precision bit for bit: BF16 / FP16 keep 16-bit
- * host state between chunk visits and run the last repetition in FP32.
+ * host state between chunk visits and run the last repetition at FP32
+ * accuracy, in the same form as the in-memory run.
  * Replaces NS::diffusion_out_of_core (hh:74, gpu:947-1129). */
 int mxb_diffusion_chunked(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
                           const float *kernel, int64_t kernel_size, int64_t repetitions,
@@ -110,7 +116,7 @@ enum {
 };
 
 /* mxb_diffusion_on_disk with a precision (mxb_precision; the 16-bit forms keep
- * 16-bit state files and run their last repetition in FP32) and flags (0 or
+ * 16-bit state files and run their last repetition at FP32 accuracy) and flags (0 or
  * MXB_ON_DISK_REFERENCE_MASK).  Replaces the same reference interface as
  * mxb_diffusion_on_disk (pyb:67-77; cpu:376-468; gpu:808-945). */
 int mxb_diffusion_on_disk_ex(const char *input_file, const float *kernel, int64_t kernel_size,

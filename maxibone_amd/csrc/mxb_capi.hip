@@ -132,7 +132,8 @@ static bool is_lowp(int precision) { return precision == MXB_BF16 || precision =
 
 // Every form needs two FP32-sized state buffers: the 16-bit forms use the
 // first half of each for their repetitions and the whole of both for the
-// final FP32 repetition; the split form holds its FP16 high plane at the
+// final repetition (FP32 state, or the split planes and the FP32 result);
+// the split form holds its FP16 high plane at the
 // start and its low plane split_lo_off(n) elements further.  The first
 // buffer also carries the mask packed to bits after the state (at byte
 // state_f32_bytes).
@@ -200,10 +201,16 @@ static int run_steps_f32(const uint8_t *d_mask, const MaskBits &bits, int64_t nz
 // 16-bit repetitions: n16 repetitions keep BF16/FP16 state (z: s0 -> s1,
 // y: s1 -> s0, x: s0 -> s1 + illumination, swap), starting from
 // (mask > 0) when init_from_mask, else from the 16-bit state already in s0.
-// With final_f32 the state is then widened to FP32 and one more repetition
-// runs in FP32 (MFMA), writing out_u16 / out_f32, so the result carries no
-// rounding of its own repetition; otherwise the 16-bit state is left where
-// *final16 points.
+// With final_f32 one more repetition then runs at FP32 accuracy, writing
+// out_u16 / out_f32, so the result carries no 16-bit rounding of its own
+// repetition: where the fused pass serves the volume, in the split form (its
+// z-pass reads the 16-bit state as the FP16 high plane, the fused x -> y
+// pass writes the outputs); otherwise, and for a lone repetition from the
+// mask (as in-memory runs of one repetition do), the state is widened to
+// FP32 for one FP32 (MFMA) repetition.  The choice depends only on the
+// volume and the 16-bit state, so chunked runs, which visit the final
+// repetition with n16 = 0, match in-memory runs bit for bit.  Without
+// final_f32 the 16-bit state is left where *final16 points.
 static int run_lowp_core(const uint8_t *d_mask, const MaskBits &bits, int64_t nz, int64_t ny,
                          int64_t nx, const float *k, int64_t K, int64_t n16, bool final_f32, bool bf,
                          bool init_from_mask, void *s0, void *s1, uint16_t *out_u16, float *out_f32,
@@ -232,6 +239,27 @@ static int run_lowp_core(const uint8_t *d_mask, const MaskBits &bits, int64_t nz
     if (!final_f32) {
         if (final16) *final16 = cur;
         return OK;
+    }
+    if (fused && !(init_from_mask && n16 == 0)) {
+        // split planes into the other buffer; the 16-bit state's buffer then
+        // takes the FP32 state when one is wanted
+        const int64_t lo = split_lo_off(n);
+        SplitIO iz;
+        iz.out_lo = lo;
+        iz.widen = prec;
+        SplitIO ex;
+        ex.in_lo = ex.out_lo = lo;
+        ex.fin_u16 = out_u16;
+        ex.fin_f32 = out_f32 != nullptr ? out_f32 : (final_state != nullptr || out_u16 == nullptr ? (float *)cur : nullptr);
+        if (split_pass_supported(cur, tmp, nullptr, bits, gz, R, iz) &&
+            xy_fused_supported(tmp, cur, bits, nz, ny, nx, R, MXB_FP32_SPLIT, ex)) {
+            MXB_HIP(timed(MXB_K_ZPASS, s, [&] { return launch_line_pass_split(cur, tmp, nullptr, bits, gz, k, R, iz, s); }));
+            MXB_HIP(timed(MXB_K_XYPASS, s, [&] {
+                return launch_xy_fused(tmp, cur, bits, nz, ny, nx, k, R, MXB_FP32_SPLIT, ex, s);
+            }));
+            if (final_state) *final_state = ex.fin_f32;
+            return OK;
+        }
     }
     // widen into the other buffer, then one FP32 repetition
     float *f0 = (float *)(cur == (uint16_t *)s0 ? s1 : s0);
@@ -394,7 +422,8 @@ static void log_v(int verbose, int level, const std::string &msg) {
 // that alignment every layer keeps its place, so 16-bit chunked results equal
 // in-memory ones bit for bit (FP32 passes do not depend on the tile place).
 // Each block of a 16-bit diffusion runs its repetitions in 16-bit state; the
-// last block runs its last repetition in FP32, as the in-memory form does.
+// last block runs its last repetition at FP32 accuracy from the 16-bit state,
+// in the form the in-memory run takes.
 //
 // ref_mask: illuminate with (u == 1.0f) of the state each chunk visit starts
 // from instead of the input mask: the reference's on-disk path re-derives its

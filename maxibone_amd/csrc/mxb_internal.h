@@ -118,6 +118,10 @@ struct SplitIO {
     float scale = 1.0f;
     float *fin_f32 = nullptr;
     uint16_t *fin_u16 = nullptr;
+    // z-pass only: the input is one 16-bit state plane (MXB_BF16 or
+    // MXB_FP16; 0 = hi + lo planes), taken as the hi plane with lo 0 (BF16
+    // values widened to FP16), so a 16-bit state enters the split form
+    int widen = 0;
 };
 
 // Split form (mxb_kernels_lowp_lds.hip, mxb_lowp_dma_body.h).  A pass reads

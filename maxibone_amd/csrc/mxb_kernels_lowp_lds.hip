@@ -26,6 +26,7 @@
 // transposed read touches are p0 + {0..3, 8..11, 16..19, 24..27} with
 // p0 = 0 mod 4; per half-wave the eight 32-byte pieces then fall in eight
 // distinct 32-byte bank groups.
+#include "maxibone_diffusion.h"
 #include "mxb_dma.h"
 #include "mxb_lowp.h"
 #include "mxb_lowp_dma_body.h"
@@ -359,31 +360,39 @@ static hipError_t launch_lowp_staged_rt(const uint16_t *in, uint16_t *out, const
 #define MXB_SPLIT_DEPTH_X 5
 #endif
 
-template <int RT, int D, bool MASK, bool ZSTAGE, int FIN>
+template <int RT, int D, bool MASK, bool ZSTAGE, int FIN, int WIDEN = 0>
 __global__ __launch_bounds__(256) void k_line_split_dma(const uint16_t *__restrict__ in,
                                                         uint16_t *__restrict__ out,
                                                         const uint8_t *__restrict__ mask,
                                                         MaskBits bits, LineGeom g,
                                                         TapsL<2 * RT + 1> taps, SplitIO io) {
     __shared__ __attribute__((aligned(16)))
-    unsigned char smem[LowpDmaSmem<RT, D, MASK, ZSTAGE, true, FIN>::BYTES];
+    unsigned char smem[LowpDmaSmem<RT, D, MASK, ZSTAGE, true, FIN, WIDEN>::BYTES];
     int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t seg_id = bid % g.nseg;
     bid /= g.nseg;
     const int64_t nlb = (g.n_lane + 63) >> 6;
-    lowp_dma_body<RT, D, false, MASK, ZSTAGE, true, FIN>(in, out, mask, bits, g, taps.hi, taps.lo, bid / nlb,
-                                                         bid % nlb, seg_id, smem, io);
+    lowp_dma_body<RT, D, false, MASK, ZSTAGE, true, FIN, WIDEN>(in, out, mask, bits, g, taps.hi, taps.lo,
+                                                                bid / nlb, bid % nlb, seg_id, smem, io);
 }
 
 // Which split-form kernel serves a pass: 0 none; 1 line-contiguous (no
 // mask, planes out); 2 position-contiguous planes out; 3 the same,
 // illuminating from mask bits; 4 + fin (fin = 1 FP32, 2 uint16, 3 both):
-// position-contiguous final outputs.
+// position-contiguous final outputs; 8 / 9 form 1 from one BF16 / FP16
+// plane (io.widen).
 static int split_form(const uint16_t *in, const uint16_t *out, const uint8_t *mask, const MaskBits &bits,
                       const LineGeom &g, const SplitIO &io) {
     if (g.n_lane % 8 != 0 || g.in_outer % 8 != 0 || g.in_ax % 8 != 0 || io.in_lo % 8 != 0 ||
         reinterpret_cast<uintptr_t>(in) % 16 != 0)
         return 0;
+    if (io.widen != 0) {
+        const bool line = mask == nullptr && io.fin_f32 == nullptr && io.fin_u16 == nullptr &&
+                          g.out_lane == 1 && g.out_outer % 8 == 0 && g.out_ax % 8 == 0 &&
+                          reinterpret_cast<uintptr_t>(out) % 16 == 0 && io.out_lo % 8 == 0;
+        if (!line || (io.widen != MXB_BF16 && io.widen != MXB_FP16)) return 0;
+        return io.widen == MXB_BF16 ? 8 : 9;
+    }
     if (mask != nullptr && bits.w == nullptr) return 0;
     const int fin = (io.fin_f32 != nullptr ? 1 : 0) + (io.fin_u16 != nullptr ? 2 : 0);
     if (fin != 0) {
@@ -426,6 +435,8 @@ static hipError_t launch_split_rt(int form, const uint16_t *in, uint16_t *out, c
     case 5: hipLaunchKernelGGL((k_line_split_dma<RT, DX, true, false, 1>), grid, block, 0, s, in, out, mask, bits, g, t, io); break;
     case 6: hipLaunchKernelGGL((k_line_split_dma<RT, DX, true, false, 2>), grid, block, 0, s, in, out, mask, bits, g, t, io); break;
     case 7: hipLaunchKernelGGL((k_line_split_dma<RT, DX, true, false, 3>), grid, block, 0, s, in, out, mask, bits, g, t, io); break;
+    case 8: hipLaunchKernelGGL((k_line_split_dma<RT, DZ, false, true, 0, 1>), grid, block, 0, s, in, out, mask, bits, g, t, io); break;
+    case 9: hipLaunchKernelGGL((k_line_split_dma<RT, DZ, false, true, 0, 2>), grid, block, 0, s, in, out, mask, bits, g, t, io); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -77,6 +77,19 @@ __device__ __forceinline__ void split_f16(float v, uint16_t &hi, uint16_t &lo) {
     lo = __builtin_bit_cast(uint16_t, (_Float16)(v - (float)h));
 }
 
+// Eight BF16 values (one MFMA operand) as FP16: exact from 2^-14, FP16's
+// smallest normal number, up to 65504; smaller magnitudes round to FP16's
+// subnormal step of 2^-24 (an error below 2^-25).
+__device__ __forceinline__ v8s bf16x8_to_f16(v8s x) {
+    v8s r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const float f = __builtin_bit_cast(float, (uint32_t)(uint16_t)x[e] << 16);
+        r[e] = __builtin_bit_cast(short, (_Float16)f);
+    }
+    return r;
+}
+
 __device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) {
     return (uint32_t)lo | ((uint32_t)hi << 16);
 }

@@ -63,11 +63,13 @@ constexpr int kMaskSlotBytes = 64 * 16;  // 64 lines x 4 pairs' 32-bit words of 
 constexpr int kStagePitchF = 36;  // floats per line row (position-contiguous FP32 epilogue)
 
 // SPLIT: two rings (high and low planes) and two 16-bit stages; FIN != 0: the
-// position-contiguous epilogue stages floats.
-template <int RT, int D, bool MASK, bool ZSTAGE, bool SPLIT = false, int FIN = 0>
+// position-contiguous epilogue stages floats; WIDEN != 0 (split form): one
+// ring, the input being one 16-bit plane (1 BF16, 2 FP16) taken as the high
+// plane with a zero low plane.
+template <int RT, int D, bool MASK, bool ZSTAGE, bool SPLIT = false, int FIN = 0, int WIDEN = 0>
 struct LowpDmaSmem {
     using S = LowpDmaShape<RT, D>;
-    static constexpr int RING = S::NSL * kSlotBytes * (SPLIT ? 2 : 1);
+    static constexpr int RING = S::NSL * kSlotBytes * (SPLIT && WIDEN == 0 ? 2 : 1);
     static constexpr int MRING = MASK ? S::NM * kMaskSlotBytes : 0;
     static constexpr int STAGE = FIN != 0 ? 64 * kStagePitchF * 4 : kStageBytesL * (SPLIT ? 2 : 1);
     static constexpr int TAPS = 2 * LowpChunks<RT>::TW * 2;  // hi and lo tables, uint16
@@ -78,7 +80,7 @@ struct LowpDmaSmem {
 // LowpDmaSmem::BYTES of LDS, `tap_hi` / `tap_lo` the 2 RT + 1 centred taps'
 // 16-bit high and low parts.  Ends with every load and store drained and a
 // barrier, so a persistent caller may reuse `smem` at once.
-template <int RT, int D, bool BF, bool MASK, bool ZSTAGE, bool SPLIT = false, int FIN = 0>
+template <int RT, int D, bool BF, bool MASK, bool ZSTAGE, bool SPLIT = false, int FIN = 0, int WIDEN = 0>
 __device__ __forceinline__ void lowp_dma_body(const uint16_t *__restrict__ in, uint16_t *__restrict__ out,
                                               const uint8_t *__restrict__ mask, const MaskBits &bits,
                                               const LineGeom &g, const uint16_t *tap_hi,
@@ -89,15 +91,17 @@ __device__ __forceinline__ void lowp_dma_body(const uint16_t *__restrict__ in, u
     using L = Lowp<BF>;
     constexpr int C = S::C, W = S::W, NSL = S::NSL, NM = S::NM;
     constexpr int TW = LowpChunks<RT>::TW;
-    constexpr int P = SPLIT ? 2 : 1;  // state planes: data loads per slot
-    constexpr int GW = P * (D - 1) + (MASK ? (D - 1) / 4 : 0);  // loads in D - 1 groups
+    constexpr bool LO = SPLIT && WIDEN == 0;  // a low plane is read
+    constexpr int P = SPLIT ? 2 : 1;          // state planes written
+    constexpr int GW = (LO ? 2 : 1) * (D - 1) + (MASK ? (D - 1) / 4 : 0);  // loads in D - 1 groups
     // stores per pair and thread
     constexpr int ST = FIN != 0 ? ((FIN & 1) ? 2 : 0) + ((FIN & 2) ? 1 : 0) : P;
     static_assert(!MASK || (D - 1) % 4 == 0, "a fixed count of mask loads per D - 1 groups");
     static_assert(!(MASK && ZSTAGE), "the illuminating pass writes positions contiguously");
     static_assert(!SPLIT || !BF, "the split form runs on FP16 MFMAs");
     static_assert(FIN == 0 || (SPLIT && !ZSTAGE), "final outputs: the split form's position-contiguous pass");
-    using M = LowpDmaSmem<RT, D, MASK, ZSTAGE, SPLIT, FIN>;
+    static_assert(WIDEN == 0 || (SPLIT && ZSTAGE && !MASK), "widening: the split form's z-pass");
+    using M = LowpDmaSmem<RT, D, MASK, ZSTAGE, SPLIT, FIN, WIDEN>;
     unsigned char *const ring = smem;
     unsigned char *const ring_lo = smem + S::NSL * kSlotBytes;  // SPLIT only
     unsigned char *const mring = smem + M::RING;
@@ -138,7 +142,7 @@ __device__ __forceinline__ void lowp_dma_body(const uint16_t *__restrict__ in, u
         const bool ok = dl_ok && slot < nslots && p >= 0 && p < n_ax;
         const char *src = ok ? dsrc + p * in_ax_b : reinterpret_cast<const char *>(g_dma_zero);
         dma16<ZSTAGE>(ring_lds + (uint32_t)((slot % NSL) * kSlotBytes), src);
-        if constexpr (SPLIT)
+        if constexpr (LO)
             dma16<ZSTAGE>(ring_lds + (uint32_t)((NSL + slot % NSL) * kSlotBytes), ok ? src + in_lo_b : src);
     };
     // ---- mask bits: lane -> pair pj + lane / 16, line 16 wv + lane % 16 (the
@@ -212,15 +216,16 @@ __device__ __forceinline__ void lowp_dma_body(const uint16_t *__restrict__ in, u
                     si = si >= NSL ? si - NSL : si;
                     part[h] = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
                         (lds_v4s *)(ring + si * kSlotBytes + roff[u][h]));
-                    if constexpr (SPLIT)
+                    if constexpr (LO)
                         plo[h] = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
                             (lds_v4s *)(ring_lo + si * kSlotBytes + roff[u][h]));
                 }
-                const v8s x = {part[0][0], part[0][1], part[0][2], part[0][3],
-                               part[1][0], part[1][1], part[1][2], part[1][3]};
+                v8s x = {part[0][0], part[0][1], part[0][2], part[0][3],
+                         part[1][0], part[1][1], part[1][2], part[1][3]};
+                if constexpr (WIDEN == 1) x = bf16x8_to_f16(x);
                 acc[u] = L::mfma(ahi[c], x, acc[u]);
                 acc[u] = L::mfma(alo[c], x, acc[u]);
-                if constexpr (SPLIT) {
+                if constexpr (LO) {
                     const v8s xl = {plo[0][0], plo[0][1], plo[0][2], plo[0][3],
                                     plo[1][0], plo[1][1], plo[1][2], plo[1][3]};
                     acc[u] = L::mfma(ahi[c], xl, acc[u]);

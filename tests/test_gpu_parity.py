@@ -287,7 +287,7 @@ def test_zero_repetitions_and_determinism(gpu):
 
 
 # ---------------------------------------------------------------- precision form
-LOWP_TOL = 2e-3  # max-abs vs the FP64 oracle (BF16/FP16 storage, FP32 last repetition)
+LOWP_TOL = 2e-3  # max-abs vs the FP64 oracle (BF16/FP16 storage, FP32-accuracy last repetition)
 
 
 def _lowp_errors(gpu, m, k32, reps, prec):
@@ -356,6 +356,28 @@ def test_flat_plane_z_pass(gpu, prec):
                 assert mx <= LOWP_TOL
             else:
                 _check_f32(gpu, m, k, 5, prec)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_precision_form_final_repetition_on_exact_values(gpu, prec):
+    """Taps 1/4, 1/2, 1/4 keep two repetitions from a 0/1 state exact: the
+    first repetition's values (multiples of 1/64) are exact 16-bit numbers,
+    and the final one, which starts from that 16-bit state (in the split
+    form's widening z-pass and fused pass where the fused pass serves the
+    volume, else in the FP32 form), is exact as well.  So the result equals
+    the FP32 oracle bit for bit, in-memory and chunked (where a chunk visit
+    runs the final repetition alone).  Shapes: standard and flat-plane z
+    layouts, and rows of 100 voxels, which the fused pass does not serve."""
+    rng = np.random.default_rng(2)
+    k = np.array([0.25, 0.5, 0.25], np.float32)
+    for shape in [(20, 24, 64), (9, 40, 136), (16, 8, 1000), (6, 30, 100)]:
+        m = (rng.random(shape) > 0.9).astype(np.uint8)
+        u_ref, q_ref = oracle.diffusion_f32(m, k, 2)
+        _say("exact final repetition", prec, shape)
+        q, u = gpu.diffusion(m, k, 2, precision=prec, f32=True)
+        assert np.array_equal(u.view(np.uint32), u_ref.view(np.uint32)) and np.array_equal(q, q_ref)
+        qc, uc = gpu.diffusion_chunked(m, k, 2, 5, 1, precision=prec, f32=True)
+        assert np.array_equal(uc.view(np.uint32), u_ref.view(np.uint32)) and np.array_equal(qc, q_ref)
 
 
 @pytest.mark.parametrize("prec", ["bf16", "fp16"])
@@ -492,7 +514,8 @@ def test_batched_random_shapes_equal_single_volume(gpu, three_pass, prec):
 @pytest.mark.parametrize("chunk_z,temporal", [(7, 1), (16, 3), (40, 2), (5, 4)])
 def test_chunked_16bit_equals_in_memory(gpu, prec, chunk_z, temporal):
     """16-bit chunked runs keep 16-bit host state between chunk visits and run
-    the last repetition in FP32: bit-equal to the in-memory 16-bit form."""
+    the last repetition at FP32 accuracy from it (T = 1: a chunk visit runs
+    it alone): bit-equal to the in-memory 16-bit form."""
     rng = np.random.default_rng(chunk_z * 31 + temporal)
     m = (rng.random((77, 33, 70)) > 0.97).astype(np.uint8)
     k = gauss_kernel(2.2).astype(np.float32)
