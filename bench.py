@@ -464,13 +464,13 @@ def bench_chunked(args, w, reps):
         "value": round(float(n) * reps / t / 1e9, 3), "unit": "Gvoxel*iter/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t * 1e3, 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
+        "dtype": "f32" if args.precision.startswith("fp32") else args.precision, "data": "synthetic",
         "config": {"workload": f"{w.name}: {w.description}", "shape": list(w.shape), "sigma": w.sigma,
                    "radius": w.radius, "repetitions": reps, "path": "chunked, host-resident state",
                    "chunk_z": args.chunked, "temporal": args.temporal,
-                   "halo": args.temporal * w.radius, "output": "uint16"},
+                   "halo": args.temporal * w.radius, "precision": args.precision, "output": "uint16"},
         "note": "PCIe-inclusive: mask, state and outputs live in host memory; the chunked driver keeps "
-                "FP32 state between chunk visits",
+                + ("16-bit" if args.precision in ("bf16", "fp16") else "FP32") + " state between chunk visits",
     }
     print(json.dumps(rec), flush=True)
 

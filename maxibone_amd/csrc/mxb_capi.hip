@@ -325,18 +325,27 @@ static int run_steps_split(const uint8_t *d_mask, const MaskBits &bits, int64_t 
                              !from_f32, out_u16, out_f32, final_state, s);
     const int64_t lo = split_lo_off(n);
     uint16_t *cur = (uint16_t *)s0, *tmp = (uint16_t *)s1;
-    if (from_f32) {
-        MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_split_state((const float *)s0, tmp, lo, n, s); }));
-        std::swap(cur, tmp);
-    } else {
-        MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_init_split(d_mask, cur, lo, n, s); }));
-    }
     const LineGeom gz = geom_z(nz, ny, nx), gy = geom_y(nz, ny, nx), gx = geom_x(nz, ny, nx);
     SplitIO io;
     io.in_lo = io.out_lo = lo;
+    // From the mask, the 0/1 state is exact in FP16: the first z-pass reads
+    // it as the high plane alone (its low plane would be zeros).
+    SplitIO iz0 = io;
+    iz0.in_lo = 0;
+    iz0.widen = MXB_FP16;
+    const bool widen0 = !from_f32 && split_pass_supported(cur, tmp, nullptr, bits, gz, R, iz0);
+    if (from_f32) {
+        MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_split_state((const float *)s0, tmp, lo, n, s); }));
+        std::swap(cur, tmp);
+    } else if (widen0) {
+        MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_init_lowp(d_mask, cur, n, false, s); }));
+    } else {
+        MXB_HIP(timed(MXB_K_OTHER, s, [&] { return launch_init_split(d_mask, cur, lo, n, s); }));
+    }
     for (int64_t t = 0; t < reps; ++t) {
         const bool last = t == reps - 1;
-        MXB_HIP(timed(MXB_K_ZPASS, s, [&] { return launch_line_pass_split(cur, tmp, nullptr, bits, gz, k, R, io, s); }));
+        const SplitIO &iz = t == 0 && widen0 ? iz0 : io;
+        MXB_HIP(timed(MXB_K_ZPASS, s, [&] { return launch_line_pass_split(cur, tmp, nullptr, bits, gz, k, R, iz, s); }));
         // the last pass's input is tmp (fused) or cur (three-pass): its other
         // buffer takes the FP32 state when one is wanted
         uint16_t *const free_buf = form == 2 ? cur : tmp;

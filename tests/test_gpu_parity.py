@@ -367,10 +367,11 @@ def test_precision_form_final_repetition_on_exact_values(gpu, prec):
     volume, else in the FP32 form), is exact as well.  So the result equals
     the FP32 oracle bit for bit, in-memory and chunked (where a chunk visit
     runs the final repetition alone).  Shapes: standard and flat-plane z
-    layouts, and rows of 100 voxels, which the fused pass does not serve."""
+    layouts, single rows and planes, and rows of 100 voxels, which the fused
+    pass does not serve."""
     rng = np.random.default_rng(2)
     k = np.array([0.25, 0.5, 0.25], np.float32)
-    for shape in [(20, 24, 64), (9, 40, 136), (16, 8, 1000), (6, 30, 100)]:
+    for shape in [(20, 24, 64), (9, 40, 136), (16, 8, 1000), (1, 1, 64), (3, 1, 8), (1, 40, 16), (6, 30, 100)]:
         m = (rng.random(shape) > 0.9).astype(np.uint8)
         u_ref, q_ref = oracle.diffusion_f32(m, k, 2)
         _say("exact final repetition", prec, shape)
