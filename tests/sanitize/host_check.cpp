@@ -86,7 +86,7 @@ static int cpu_checks() {
     std::vector<float> u(m.size());
 
     EXPECT(mxb_version() != nullptr && std::strncmp(mxb_version(), "maxibone_amd", 12) == 0);
-    EXPECT(mxb_state_bytes(MXB_FP32, 4, 5, 6) == 512 + 80);
+    EXPECT(mxb_state_bytes(MXB_FP32, 4, 5, 6) == 512 + 2048 * 4 + 80);  // + the split form's low-plane gap
     EXPECT(mxb_state_bytes(1024, 4, 5, 6) == -1);
     EXPECT(mxb_state_bytes(MXB_FP32, 4, -5, 6) == -1);
     EXPECT(std::strlen(mxb_last_error()) > 0);

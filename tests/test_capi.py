@@ -27,10 +27,13 @@ def test_library_exports_every_declared_symbol():
 
 def test_state_bytes():
     L = capi.lib()
-    # 480 bytes of FP32 state rounded to 512, then 4 * 5 rows of one bit word
-    assert L.mxb_state_bytes(capi.PRECISION["fp32"], 4, 5, 6) == 512 + 80
-    assert L.mxb_state_bytes(capi.PRECISION["bf16"], 4, 5, 6) == 512 + 80  # FP32 last repetition
-    assert L.mxb_state_bytes(capi.PRECISION["fp32"], 2, 3, 33) == (2 * 3 * 33 * 4 + 255) // 256 * 256 + 2 * 3 * 2 * 4
+    # 120 voxels rounded to 128, plus the 2048-element gap before the split
+    # form's low plane, in FP32 bytes; then 4 * 5 rows of one bit word
+    gap = 2048 * 4
+    assert L.mxb_state_bytes(capi.PRECISION["fp32"], 4, 5, 6) == 512 + gap + 80
+    assert L.mxb_state_bytes(capi.PRECISION["bf16"], 4, 5, 6) == 512 + gap + 80  # FP32 last repetition
+    assert L.mxb_state_bytes(capi.PRECISION["fp32"], 2, 3, 33) == ((2 * 3 * 33 + 127) // 128 * 128 * 4 + gap
+                                                                    + 2 * 3 * 2 * 4)
     assert L.mxb_state_bytes(1024, 4, 5, 6) == -1   # unknown precision
     assert L.mxb_state_bytes(capi.PRECISION["fp32"], 4, -5, 6) == -1
 
