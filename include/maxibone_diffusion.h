@@ -176,10 +176,12 @@ int mxb_batch_destroy(mxb_batch_plan *plan);
 
 /* ---- device-resident form (inputs already in HBM) -------------------------- */
 
-/* Bytes of one state buffer for `precision` (two are needed): the FP32 state
- * rounded up to 256 bytes, then room for the mask packed to bits (one 32-bit
- * word per 32 voxels of an x row), which the first buffer carries.  -1 (and
- * mxb_last_error set) for an unknown precision or a negative extent. */
+/* Bytes of one state buffer for `precision` (two are needed): 4 bytes for
+ * each of the voxel count rounded up to 128 plus 2048 (the split form's low
+ * plane starts 4 KB past its high plane's end), then room for the mask packed
+ * to bits (one 32-bit word per 32 voxels of an x row), which the first buffer
+ * carries.  -1 (and mxb_last_error set) for an unknown precision or a
+ * negative extent. */
 int64_t mxb_state_bytes(int precision, int64_t nz, int64_t ny, int64_t nx);
 
 /* Run the diffusion on device buffers, enqueued on `stream` (a hipStream_t;
