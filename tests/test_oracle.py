@@ -248,6 +248,29 @@ def test_quantise_closed_form():
     assert oracle.quantise_u16(np.float32([1.0]))[0] == 65535
 
 
+def test_dyadic_taps_keep_two_repetitions_exact():
+    """The premise of the GPU test of the 16-bit forms' final repetition: with
+    taps 1/4, 1/2, 1/4 and a 0/1 start, the first repetition's values are
+    multiples of 1/64 with at most 7 significant bits (exact in BF16), and two
+    repetitions stay exact in FP32 (the FP32 oracle equals the FP64 one bit
+    for bit), so every arithmetic form must return the FP32 oracle's bits."""
+    rng = np.random.default_rng(2)
+    k = np.array([0.25, 0.5, 0.25], np.float32)
+    for shape in [(20, 24, 64), (9, 40, 136), (1, 1, 64), (3, 1, 8)]:
+        m = (rng.random(shape) > 0.9).astype(np.uint8)
+        u1, _ = oracle.diffusion_f32(m, k, 1)
+        scaled = u1.astype(np.float64) * 64
+        assert np.array_equal(scaled, np.round(scaled))
+        bf16 = (u1.view(np.uint32) & np.uint32(0xFFFF0000)).view(np.float32)  # truncation to BF16
+        assert np.array_equal(bf16, u1)
+        u2, _ = oracle.diffusion_f32(m, k, 2)
+        assert np.array_equal(u2.astype(np.float64), oracle.diffusion_f64(m, k, 2))
+    # a Gaussian's taps are not dyadic: there the two oracles differ
+    k = gauss_kernel(1.0).astype(np.float32)
+    m = (rng.random((20, 24, 64)) > 0.9).astype(np.uint8)
+    assert not np.array_equal(oracle.diffusion_f32(m, k, 2)[0].astype(np.float64), oracle.diffusion_f64(m, k, 2))
+
+
 @pytest.mark.parametrize("gz", [1, 4, 7, 30])
 def test_chunked_true_mask_equals_in_memory(gz):
     rng = np.random.default_rng(gz)
