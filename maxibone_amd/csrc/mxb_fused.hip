@@ -445,7 +445,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT ? 1 :
                 if constexpr (SPLIT) {
                     uint16_t hv[4], lv[4];
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) split_f16(acc[i][e] * io.scale, hv[e], lv[e]);
+                    for (int e = 0; e < 4; ++e) split_f16_scaled(acc[i][e], io.scale, hv[e], lv[e]);
                     *reinterpret_cast<uint2 *>(slot + boff) = make_uint2(pack2(hv[0], hv[1]), pack2(hv[2], hv[3]));
                     *reinterpret_cast<uint2 *>(slot + W * SLOT + boff) =
                         make_uint2(pack2(lv[0], lv[1]), pack2(lv[2], lv[3]));
@@ -531,7 +531,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT ? 1 :
                         *reinterpret_cast<float *>(west + row * S::WPF + 4 * col) = acc[0][u][e] * io.scale;
                     } else if constexpr (SPLIT) {
                         uint16_t hv, lv;
-                        split_f16(acc[0][u][e] * io.scale, hv, lv);
+                        split_f16_scaled(acc[0][u][e], io.scale, hv, lv);
                         *reinterpret_cast<uint16_t *>(west + row * S::WP16 + 2 * col) = hv;
                         *reinterpret_cast<uint16_t *>(west + 32 * S::WP16 + row * S::WP16 + 2 * col) = lv;
                     } else {
@@ -579,7 +579,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT ? 1 :
                             acc[gq][u][e] * io.scale;
                     } else if constexpr (SPLIT) {
                         uint16_t hv, lv;
-                        split_f16(acc[gq][u][e] * io.scale, hv, lv);
+                        split_f16_scaled(acc[gq][u][e], io.scale, hv, lv);
                         const int so = stage_off<S::ALIAS>(row, 2 * xc, S::SPITCH16);
                         *reinterpret_cast<uint16_t *>(est + so) = hv;
                         *reinterpret_cast<uint16_t *>(est + 32 * S::SPITCH16 + so) = lv;

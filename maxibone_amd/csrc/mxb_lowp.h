@@ -90,6 +90,17 @@ __device__ __forceinline__ v8s bf16x8_to_f16(v8s x) {
     return r;
 }
 
+// split_f16(a * scale) for a power-of-two `scale`, where the product is
+// exact: the low part's difference a * scale - hi comes from one FMA, which
+// rounds the same exact value once, so the parts equal split_f16's (one
+// instruction fewer than multiplying first).
+__device__ __forceinline__ void split_f16_scaled(float a, float scale, uint16_t &hi, uint16_t &lo) {
+#pragma clang fp contract(off)
+    const _Float16 h = (_Float16)(a * scale);
+    hi = __builtin_bit_cast(uint16_t, h);
+    lo = __builtin_bit_cast(uint16_t, (_Float16)__builtin_fmaf(a, scale, -(float)h));
+}
+
 __device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) {
     return (uint32_t)lo | ((uint32_t)hi << 16);
 }

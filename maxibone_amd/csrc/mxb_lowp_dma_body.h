@@ -261,7 +261,7 @@ __device__ __forceinline__ void lowp_dma_body(const uint16_t *__restrict__ in, u
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     uint16_t hi, lo;
-                    split_f16(acc[u][r] * io.scale, hi, lo);
+                    split_f16_scaled(acc[u][r], io.scale, hi, lo);
                     const int o = (16 * u + 4 * kg + r) * kStagePitchZ + (16 * wv + col) * 2;
                     *reinterpret_cast<uint16_t *>(stage + o) = hi;
                     *reinterpret_cast<uint16_t *>(stage_lo + o) = lo;
@@ -342,7 +342,7 @@ __device__ __forceinline__ void lowp_dma_body(const uint16_t *__restrict__ in, u
                 if constexpr (SPLIT) {
                     uint16_t h[4], l[4];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) split_f16(acc[u][r] * io.scale, h[r], l[r]);
+                    for (int r = 0; r < 4; ++r) split_f16_scaled(acc[u][r], io.scale, h[r], l[r]);
                     const int o = (16 * wv + col) * kStagePitchV + (16 * u + 4 * kg) * 2;
                     *reinterpret_cast<uint2 *>(stage + o) = make_uint2(pack2(h[0], h[1]), pack2(h[2], h[3]));
                     *reinterpret_cast<uint2 *>(stage_lo + o) = make_uint2(pack2(l[0], l[1]), pack2(l[2], l[3]));
