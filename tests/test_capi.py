@@ -31,7 +31,7 @@ def test_state_bytes():
     # form's low plane, in FP32 bytes; then 4 * 5 rows of one bit word
     gap = 2048 * 4
     assert L.mxb_state_bytes(capi.PRECISION["fp32"], 4, 5, 6) == 512 + gap + 80
-    assert L.mxb_state_bytes(capi.PRECISION["bf16"], 4, 5, 6) == 512 + gap + 80  # FP32 last repetition
+    assert L.mxb_state_bytes(capi.PRECISION["bf16"], 4, 5, 6) == 512 + gap + 80  # FP32-sized: the last repetition
     assert L.mxb_state_bytes(capi.PRECISION["fp32"], 2, 3, 33) == ((2 * 3 * 33 + 127) // 128 * 128 * 4 + gap
                                                                     + 2 * 3 * 2 * 4)
     assert L.mxb_state_bytes(1024, 4, 5, 6) == -1   # unknown precision
